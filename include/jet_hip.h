@@ -1,0 +1,184 @@
+/*
+ * jet_hip.h -- C ABI of libjet_hip.so, the MI355X (gfx950) pairwise
+ * tensor-contraction path of Jet.
+ *
+ * All entry points are extern "C", take plain pointers and sizes, return an
+ * int status (JET_HIP_SUCCESS == 0) and never throw.  On failure the message
+ * is available from jet_hip_last_error() (thread-local), formatted like the
+ * reference's Jet::Exception text (include/jet/Abort.hpp:90-97) so that a C++
+ * wrapper can rethrow it unchanged.
+ *
+ * Tensors are dense complex arrays in device memory.  Modes are int32 labels;
+ * every entry point takes a layout flag:
+ *   JET_HIP_ROW_MAJOR -- last mode fastest, the convention of Jet::Tensor
+ *                        (include/jet/Tensor.hpp:94-96);
+ *   JET_HIP_COL_MAJOR -- first mode fastest, the convention of
+ *                        Jet::CudaTensor (include/jet/CudaTensorHelpers.hpp:117-127).
+ *
+ * Reentrancy: every function may be called concurrently from several host
+ * threads (Taskflow workers call ContractTensors concurrently,
+ * include/jet/TaskBasedContractor.hpp:382-394).  Work is ordered only by the
+ * hipStream_t passed in (NULL = the context's stream); plans are immutable
+ * after creation.
+ */
+#ifndef JET_HIP_H
+#define JET_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JET_HIP_VERSION_MAJOR 0
+#define JET_HIP_VERSION_MINOR 1
+#define JET_HIP_VERSION_PATCH 0
+
+typedef enum {
+    JET_HIP_SUCCESS = 0,
+    JET_HIP_ERROR_INVALID_VALUE = 1,
+    JET_HIP_ERROR_NOT_SUPPORTED = 2,
+    JET_HIP_ERROR_OUT_OF_MEMORY = 3,
+    JET_HIP_ERROR_DEVICE = 4,
+    JET_HIP_ERROR_INTERNAL = 5
+} jet_hip_status_t;
+
+typedef enum {
+    JET_HIP_C64 = 0,  /* std::complex<float>  / cuComplex       */
+    JET_HIP_C128 = 1  /* std::complex<double> / cuDoubleComplex */
+} jet_hip_dtype_t;
+
+typedef enum {
+    JET_HIP_ROW_MAJOR = 0,
+    JET_HIP_COL_MAJOR = 1
+} jet_hip_layout_t;
+
+/* Arithmetic of the complex GEMM inside a contraction. */
+typedef enum {
+    JET_HIP_PREC_DEFAULT = 0, /* FP32 for C64, FP64 for C128 (the reference's
+                                 CUTENSOR_COMPUTE_32F / _64F,
+                                 include/jet/CudaTensor.hpp:494-502)          */
+    JET_HIP_PREC_FP32 = 1,    /* 4M complex GEMM on FP32 MFMA                  */
+    JET_HIP_PREC_BF16X3 = 2,  /* 3M complex GEMM, each FP32 operand split into
+                                 hi+lo BF16, three BF16 MFMA products per real
+                                 product, FP32 accumulation.  C64 only.  Steps
+                                 the plan finds HBM-bound run on FP32 MFMA.   */
+    JET_HIP_PREC_FP64 = 3     /* 4M complex GEMM on FP64 MFMA (C128 only)      */
+} jet_hip_precision_t;
+
+typedef struct jet_hip_ctx_s *jet_hip_ctx_t;
+typedef struct jet_hip_plan_s *jet_hip_plan_t;
+
+/* ---- library / errors -------------------------------------------------- */
+
+int jet_hip_get_version(int *major, int *minor, int *patch);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *jet_hip_last_error(void);
+
+/* ---- context, memory, streams (replaces the cudaMalloc / cudaMemcpy /
+ *      CudaScopedDevice plumbing of include/jet/CudaTensor.hpp:145-214,
+ *      358-394 and include/jet/CudaTensorHelpers.hpp:187-231) ------------- */
+
+int jet_hip_ctx_create(int device, jet_hip_ctx_t *ctx);
+int jet_hip_ctx_destroy(jet_hip_ctx_t ctx);
+int jet_hip_ctx_device(jet_hip_ctx_t ctx, int *device);
+/* The context's own (non-blocking, default-priority) stream. */
+int jet_hip_ctx_stream(jet_hip_ctx_t ctx, void **stream);
+
+/* Pool allocation, ordered on the context stream and synchronized before
+ * return (usable from any stream).  jet_hip_free is ordered after the work
+ * already queued on the context stream; work on other streams that uses the
+ * buffer must be synchronized first. */
+int jet_hip_malloc(jet_hip_ctx_t ctx, size_t bytes, void **ptr);
+int jet_hip_free(jet_hip_ctx_t ctx, void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device.  Synchronous when
+ * stream is NULL and async == 0. */
+int jet_hip_memcpy(jet_hip_ctx_t ctx, void *dst, const void *src, size_t bytes,
+                   int kind, void *stream, int async);
+int jet_hip_memset(jet_hip_ctx_t ctx, void *dst, int value, size_t bytes,
+                   void *stream);
+int jet_hip_stream_synchronize(jet_hip_ctx_t ctx, void *stream);
+int jet_hip_device_synchronize(jet_hip_ctx_t ctx);
+
+/* ---- permutation (replaces cutensorPermutation in CudaTensor::Transpose,
+ *      include/jet/CudaTensor.hpp:701-813, and the CPU
+ *      Permuter<Backend>::Transpose, include/jet/permute/Permuter.hpp:49-119)
+ *
+ * out has modes out_modes (a permutation of in_modes); extents are those of
+ * `in` in in_modes order.  in and out must not overlap.  Any rank <= 64 and
+ * any extents; power-of-two extents take the LDS-tiled bit-permutation
+ * kernel.  Identity permutations are a device copy.                       */
+int jet_hip_permute(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
+                    jet_hip_layout_t layout, int rank, const int32_t *in_modes,
+                    const int64_t *extents, const int32_t *out_modes,
+                    const void *in, void *out, void *stream);
+
+/* ---- contraction (replaces CudaTensor::GetCudaContractionPlan /
+ *      ContractTensorsWithoutAllocation / CudaContractionPlan,
+ *      include/jet/CudaTensor.hpp:470-669)
+ *
+ * C[modes_c] = sum over modes in A and B but not C of A[modes_a]*B[modes_b].
+ * Modes in A, B and C are batch modes; modes only in A (resp. B) must be in
+ * C.  rank_c/modes_c may describe any order; ContractTensors' own order is
+ * left ++ right (include/jet/Tensor.hpp:714-745).  No conjugation.         */
+int jet_hip_contraction_plan_create(
+    jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, jet_hip_precision_t precision,
+    jet_hip_layout_t layout, int rank_a, const int32_t *modes_a,
+    const int64_t *extents_a, int rank_b, const int32_t *modes_b,
+    const int64_t *extents_b, int rank_c, const int32_t *modes_c,
+    jet_hip_plan_t *plan);
+/* Bytes of device workspace jet_hip_contract needs for this plan. */
+int jet_hip_plan_workspace_size(jet_hip_plan_t plan, size_t *bytes);
+/* Algorithmic real flops (8*M*N*K*batch) and the plan's kernel choice. */
+int jet_hip_plan_info(jet_hip_plan_t plan, double *flops, int64_t *m,
+                      int64_t *n, int64_t *k, int64_t *batch,
+                      int *gemm_kernel);
+int jet_hip_plan_destroy(jet_hip_plan_t plan);
+/* workspace may be NULL: the context's pool then provides it (stream-
+ * ordered).  a, b, c must not overlap. */
+int jet_hip_contract(jet_hip_plan_t plan, const void *a, const void *b,
+                     void *c, void *workspace, size_t workspace_bytes,
+                     void *stream);
+
+/* ---- elementwise (replaces cutensorElementwiseBinary in
+ *      CudaTensor::AddTensors, include/jet/CudaTensor.hpp:38-137, and
+ *      Tensor::AddTensors, include/jet/Tensor.hpp:412-454)
+ * c[modes_a] = a[modes_a] + b[modes_b] with b transposed to a's order.   */
+int jet_hip_add(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
+                jet_hip_layout_t layout, int rank, const int32_t *modes_a,
+                const int64_t *extents_a, const int32_t *modes_b,
+                const void *a, const void *b, void *c, void *stream);
+
+/* ---- slicing (replaces CudaTensor::SliceIndex / Tensor::SliceIndex,
+ *      include/jet/CudaTensor.hpp:865-905, include/jet/Tensor.hpp:493-526)
+ * out = in with mode `axis` fixed at `value`; other modes keep their order. */
+int jet_hip_slice(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
+                  jet_hip_layout_t layout, int rank, const int64_t *extents,
+                  int axis, int64_t value, const void *in, void *out,
+                  void *stream);
+
+/* ---- seeded fill (replaces curandGenerateUniform in
+ *      CudaTensor::FillRandom, include/jet/CudaTensor.hpp:419-446).
+ * Counter-based (splitmix64) so that the same seed gives the same values on
+ * the CPU: element i, part p (0 re, 1 im) uses counter 2*i+p.
+ * distribution 0: uniform [0,1) ; 1: uniform [-1,1) ; 2: normal N(0, 1/2)
+ * (re and im each variance 1/2).                                          */
+int jet_hip_fill_random(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, size_t n,
+                        uint64_t seed, int distribution, void *out,
+                        void *stream);
+
+/* ---- FP64 compensated reduction of per-slice results (the cross-slice
+ *      sum of TaskBasedContractor::AddReductionTask,
+ *      include/jet/TaskBasedContractor.hpp:258-280).
+ * out (device, 2 doubles: re, im) = Neumaier sum of n complex values.     */
+int jet_hip_reduce_compensated(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
+                               size_t n, const void *x, double *out,
+                               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JET_HIP_H */

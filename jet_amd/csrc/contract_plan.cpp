@@ -1,0 +1,189 @@
+// contract_plan.cpp -- host-side planning of one pairwise contraction.
+#include "contract_plan.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace jet_hip {
+
+namespace {
+
+int find(const std::vector<int32_t> &v, int32_t x)
+{
+    for (size_t i = 0; i < v.size(); i++)
+        if (v[i] == x) return (int)i;
+    return -1;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Permutation plan taking `from` (with extents) to `to`; the returned plan is
+// kCopy when the two orders give the same memory layout.
+PermutePlan plan_reorder(const std::vector<int32_t> &from,
+                         const std::vector<int64_t> &ext,
+                         const std::vector<int32_t> &to)
+{
+    std::vector<int32_t> perm(to.size());
+    for (size_t i = 0; i < to.size(); i++) perm[i] = find(from, to[i]);
+    return make_permute_plan((int)from.size(), ext.data(), perm.data());
+}
+
+} // namespace
+
+int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
+                       int64_t batch)
+{
+    const bool mfma_shape = M >= 32 && N >= 16 && K >= 16;
+    if (mfma_shape) {
+        if (mode == 2) return kGemmF64Mfma;
+        // Split-BF16 pays only where the step is bound by MFMA throughput
+        // (every dimension large enough to reuse operands from LDS).
+        if (mode == 1 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x3;
+        return kGemmF32Mfma;
+    }
+    if (K > 256 && batch * M * N <= 65536) return kGemmSmallRed;
+    return kGemmSmall;
+}
+
+ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
+                                 int ra, const int32_t *modes_a,
+                                 const int64_t *ext_a, int rb,
+                                 const int32_t *modes_b, const int64_t *ext_b,
+                                 int rc, const int32_t *modes_c)
+{
+    JH_REQUIRE(elem_bytes == 8 || elem_bytes == 16, "Unsupported data type.");
+    JH_REQUIRE(ra >= 0 && rb >= 0 && rc >= 0 && ra <= kMaxRank && rb <= kMaxRank &&
+                   rc <= kMaxRank,
+               "Tensor rank must be between 0 and 64.");
+    JH_REQUIRE(!(mode == 1 && elem_bytes != 8), "Split-BF16 needs complex64 data.");
+    JH_REQUIRE(!(mode == 2 && elem_bytes != 16), "FP64 MFMA needs complex128 data.");
+    JH_REQUIRE(!(mode == 0 && elem_bytes != 8), "FP32 MFMA needs complex64 data.");
+
+    // Row-major views (a column-major tensor is the row-major tensor with its
+    // modes reversed, CudaTensor.hpp:312-323).
+    std::vector<int32_t> A(modes_a, modes_a + ra), B(modes_b, modes_b + rb), C(modes_c, modes_c + rc);
+    std::vector<int64_t> EA(ext_a, ext_a + ra), EB(ext_b, ext_b + rb);
+    if (layout == 1) {
+        std::reverse(A.begin(), A.end());
+        std::reverse(B.begin(), B.end());
+        std::reverse(C.begin(), C.end());
+        std::reverse(EA.begin(), EA.end());
+        std::reverse(EB.begin(), EB.end());
+    }
+    for (int i = 0; i < ra; i++) {
+        JH_REQUIRE(EA[i] >= 1, "Tensor extents must be positive.");
+        JH_REQUIRE(find(A, A[i]) == i, "Duplicate indices found in the first operand.");
+    }
+    for (int i = 0; i < rb; i++) {
+        JH_REQUIRE(EB[i] >= 1, "Tensor extents must be positive.");
+        JH_REQUIRE(find(B, B[i]) == i, "Duplicate indices found in the second operand.");
+        const int ia = find(A, B[i]);
+        JH_REQUIRE(ia < 0 || EA[ia] == EB[i], "Shared index has different extents.");
+    }
+    for (int i = 0; i < rc; i++) {
+        JH_REQUIRE(find(C, C[i]) == i, "Duplicate indices found in the result.");
+        JH_REQUIRE(find(A, C[i]) >= 0 || find(B, C[i]) >= 0,
+                   "Result index is not an index of either operand.");
+    }
+    for (int i = 0; i < ra; i++)
+        JH_REQUIRE(find(B, A[i]) >= 0 || find(C, A[i]) >= 0,
+                   "Index of the first operand is neither contracted nor kept.");
+    for (int i = 0; i < rb; i++)
+        JH_REQUIRE(find(A, B[i]) >= 0 || find(C, B[i]) >= 0,
+                   "Index of the second operand is neither contracted nor kept.");
+
+    auto ext_of = [&](int32_t m) -> int64_t {
+        const int ia = find(A, m);
+        return ia >= 0 ? EA[ia] : EB[find(B, m)];
+    };
+
+    std::vector<int32_t> batch, left, right, common_a, common_b;
+    for (int32_t m : C) {
+        const bool in_a = find(A, m) >= 0, in_b = find(B, m) >= 0;
+        if (in_a && in_b)
+            batch.push_back(m);
+        else if (in_a)
+            left.push_back(m);
+        else
+            right.push_back(m);
+    }
+    for (int32_t m : A)
+        if (find(B, m) >= 0 && find(C, m) < 0) common_a.push_back(m);
+    for (int32_t m : B)
+        if (find(A, m) >= 0 && find(C, m) < 0) common_b.push_back(m);
+
+    ContractionDesc d;
+    d.elem_bytes = elem_bytes;
+    d.mode = mode;
+    for (auto m : batch) d.batch *= ext_of(m);
+    for (auto m : left) d.M *= ext_of(m);
+    for (auto m : right) d.N *= ext_of(m);
+    for (auto m : common_a) d.K *= ext_of(m);
+    for (auto e : EA) d.size_a *= e;
+    for (auto e : EB) d.size_b *= e;
+    d.size_c = d.batch * d.M * d.N;
+
+    auto cat = [](std::vector<int32_t> x, const std::vector<int32_t> &y,
+                  const std::vector<int32_t> &z) {
+        x.insert(x.end(), y.begin(), y.end());
+        x.insert(x.end(), z.begin(), z.end());
+        return x;
+    };
+    // Common-mode order: A's (the reference's choice) or B's, whichever
+    // leaves fewer elements to permute.
+    PermutePlan best_a, best_b;
+    int64_t best_cost = -1;
+    for (int opt = 0; opt < 2; opt++) {
+        const auto &common = opt == 0 ? common_a : common_b;
+        PermutePlan pa = plan_reorder(A, EA, cat(batch, left, common));
+        PermutePlan pb = plan_reorder(B, EB, cat(batch, right, common));
+        const int64_t cost = (pa.kind != PermutePlan::kCopy ? d.size_a : 0) +
+                             (pb.kind != PermutePlan::kCopy ? d.size_b : 0);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best_a = pa;
+            best_b = pb;
+        }
+    }
+    d.pa = best_a;
+    d.pb = best_b;
+    d.need_a = d.pa.kind != PermutePlan::kCopy;
+    d.need_b = d.pb.kind != PermutePlan::kCopy;
+    {
+        const std::vector<int32_t> ct = cat(batch, left, right);
+        std::vector<int64_t> ect(ct.size());
+        for (size_t i = 0; i < ct.size(); i++) ect[i] = ext_of(ct[i]);
+        d.pc = plan_reorder(ct, ect, C);
+        d.need_c = d.pc.kind != PermutePlan::kCopy;
+    }
+
+    d.gemm_kernel = choose_gemm_kernel(mode, d.M, d.N, d.K, d.batch);
+    d.flops = 8.0 * double(d.batch) * double(d.M) * double(d.N) * double(d.K);
+
+    size_t off = 0;
+    const size_t eb = (size_t)elem_bytes;
+    if (d.need_a) {
+        d.off_at = off;
+        off = align256(off + eb * (size_t)d.size_a);
+    }
+    if (d.need_b) {
+        d.off_bt = off;
+        off = align256(off + eb * (size_t)d.size_b);
+    }
+    if (d.need_c) {
+        d.off_ct = off;
+        off = align256(off + eb * (size_t)d.size_c);
+    }
+    if (d.gemm_kernel == kGemmBf16x3) {
+        d.split_a_plane = split_plane_elems(d.batch, d.M, d.K);
+        d.split_b_plane = split_plane_elems(d.batch, d.N, d.K);
+        d.off_sa = off;
+        off = align256(off + 6 * 2 * (size_t)d.split_a_plane);
+        d.off_sb = off;
+        off = align256(off + 6 * 2 * (size_t)d.split_b_plane);
+    }
+    d.workspace = off;
+    return d;
+}
+
+} // namespace jet_hip

@@ -1,0 +1,49 @@
+// contract_plan.hpp -- host-side planning of one pairwise contraction.
+//
+// The mode algebra follows Tensor::ContractTensors (Tensor.hpp:708-752) and
+// CudaTensor::GetCudaContractionPlan (CudaTensor.hpp:483-644): modes shared by
+// A and B and absent from C are summed, modes of one operand only are free,
+// and (an extension used by the batched slice loop) modes in A, B and C are
+// batch modes.  The plan maps the contraction onto
+//     permute(A -> At[batch][left][common]),
+//     permute(B -> Bt[batch][right][common]),
+//     C' [batch][left][right] = At . Bt^T   (batched NT complex GEMM),
+//     permute(C' -> C)
+// choosing the order of the batch/left/right modes from C and of the common
+// modes from whichever operand avoids the larger permutation; every
+// permutation that is a no-op in memory is skipped.
+//
+// Pure C++ so that the test suite can check plans on the CPU.
+#pragma once
+
+#include "gemm.hpp"
+#include "permute_plan.hpp"
+
+#include <cstddef>
+#include <cstdint>
+
+namespace jet_hip {
+
+struct ContractionDesc {
+    int elem_bytes = 8; // 8 complex64, 16 complex128
+    int mode = 0;       // 0 fp32, 1 bf16x3, 2 fp64
+    int64_t batch = 1, M = 1, N = 1, K = 1;
+    int64_t size_a = 1, size_b = 1, size_c = 1;
+    bool need_a = false, need_b = false, need_c = false;
+    PermutePlan pa, pb, pc;
+    int gemm_kernel = kGemmSmall;
+    // Workspace layout (bytes, 256-aligned offsets).
+    size_t off_at = 0, off_bt = 0, off_ct = 0, off_sa = 0, off_sb = 0;
+    int64_t split_a_plane = 0, split_b_plane = 0;
+    size_t workspace = 0;
+    double flops = 0; // 8 * batch * M * N * K
+};
+
+// layout: 0 row-major, 1 column-major.  mode as in ContractionDesc.
+ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
+                                 int ra, const int32_t *modes_a,
+                                 const int64_t *ext_a, int rb,
+                                 const int32_t *modes_b, const int64_t *ext_b,
+                                 int rc, const int32_t *modes_c);
+
+} // namespace jet_hip

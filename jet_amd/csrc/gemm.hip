@@ -1,0 +1,527 @@
+// gemm.hip -- complex GEMM kernels on CDNA4 MFMA (gfx950).
+//
+// Three arithmetic modes, all computing C[b][m][n] = sum_k A[b][m][k] B[b][n][k]
+// (no conjugation, alpha = 1, beta = 0):
+//
+//  * FP32 / FP64 "4M" (cgemm_embed_kernel): the complex product is the real
+//    product of A viewed as an M x 2K real matrix (re/im interleaved along k)
+//    and the 2K x 2N real matrix B' with B'[2k][2n]=br, B'[2k][2n+1]=bi,
+//    B'[2k+1][2n]=-bi, B'[2k+1][2n+1]=br.  Its output is C with re/im
+//    interleaved, so A and C are used in their native layout and B' is formed
+//    in registers.  v_mfma_{f32,f64}_16x16x4 each cover 16 rows x 8 complex
+//    columns x 2 complex k.
+//
+//  * split-BF16 "3M" (cgemm_bf16x3_kernel): T1 = Ar.Br, T2 = Ai.Bi,
+//    T3 = (Ar+Ai).(Br+Bi), Cr = T1 - T2, Ci = T3 - T1 - T2, where every real
+//    product X.Y is Xh.Yh + Xh.Yl + Xl.Yh with X = Xh + Xl split into two
+//    BF16 numbers (the Xl.Yl term, ~2^-18 relative, is dropped) and the sums
+//    accumulate in FP32 on v_mfma_f32_16x16x32_bf16.  Operands come
+//    pre-split (split_bf16_kernel) as six k-contiguous planes.
+//
+//  * VALU kernels for shapes too small or skinny for MFMA tiles.
+#include "common.hpp"
+#include "gemm.hpp"
+
+#include <algorithm>
+
+namespace jet_hip {
+
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <class R> struct Vec;
+template <> struct Vec<float> {
+    using c = float2;
+    using acc = f32x4;
+};
+template <> struct Vec<double> {
+    using c = double2;
+    using acc = f64x4;
+};
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma4(double a, double b, f64x4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// 4M complex GEMM on FP32 / FP64 MFMA.
+// Block tile BM x BN complex outputs, BK complex k per LDS stage, 4 waves in
+// a WM x WN grid, double-buffered LDS with register prefetch.
+// Lane l of a 16x16x4 MFMA holds A[l%16][l/16] and B[l/16][l%16] and gets
+// D[4(l/16)+r][l%16]; here the four k-slots of four consecutive MFMAs cover
+// 8 complex k: lane slot q=l/16 of MFMA g is real k 4q+g, i.e. complex k
+// 2q+g/2, real part for even g and imaginary part for odd g.
+// ---------------------------------------------------------------------------
+template <class R, int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__(256) void cgemm_embed_kernel(GemmArgs g)
+{
+    using C2 = typename Vec<R>::c;
+    using Acc = typename Vec<R>::acc;
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int FM = WTM / 16, FN = WTN / 8;
+    constexpr int PAD = sizeof(R) == 4 ? 2 : 1;
+    constexpr int LROW = BK + PAD;
+    constexpr int A_PER = BM * BK / 256;
+    constexpr int B_PER = BN * BK / 256;
+    static_assert(WM * WN == 4, "4 waves");
+    static_assert(BK % 8 == 0 && A_PER * 256 == BM * BK && B_PER * 256 == BN * BK, "tile");
+    __shared__ C2 As[2][BM * LROW];
+    __shared__ C2 Bs[2][BN * LROW];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int64_t bz = blockIdx.z;
+    const int64_t m0 = int64_t(blockIdx.x) * BM, n0 = int64_t(blockIdx.y) * BN;
+    const C2 *__restrict__ A = static_cast<const C2 *>(g.A) + bz * g.sA;
+    const C2 *__restrict__ B = static_cast<const C2 *>(g.B) + bz * g.sB;
+    R *__restrict__ C = reinterpret_cast<R *>(static_cast<C2 *>(g.C) + bz * g.sC);
+    const int64_t M = g.M, N = g.N, K = g.K;
+
+    Acc acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = Acc{0, 0, 0, 0};
+
+    C2 ra[A_PER], rb[B_PER];
+    const C2 zero = {R(0), R(0)};
+    auto gload = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < A_PER; i++) {
+            const int e = tid + i * 256, r = e / BK, kk = e % BK;
+            const int64_t m = m0 + r, k = k0 + kk;
+            ra[i] = (m < M && k < K) ? A[m * g.lda + k] : zero;
+        }
+#pragma unroll
+        for (int i = 0; i < B_PER; i++) {
+            const int e = tid + i * 256, r = e / BK, kk = e % BK;
+            const int64_t n = n0 + r, k = k0 + kk;
+            rb[i] = (n < N && k < K) ? B[n * g.ldb + k] : zero;
+        }
+    };
+    auto swrite = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < A_PER; i++) {
+            const int e = tid + i * 256;
+            As[buf][(e / BK) * LROW + e % BK] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < B_PER; i++) {
+            const int e = tid + i * 256;
+            Bs[buf][(e / BK) * LROW + e % BK] = rb[i];
+        }
+    };
+
+    const int64_t nk = (K + BK - 1) / BK;
+    gload(0);
+    swrite(0);
+    __syncthreads();
+    const int q = lane >> 4, j = lane & 15, t = lane & 1;
+    for (int64_t kt = 0; kt < nk; kt++) {
+        const int buf = int(kt & 1);
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+        for (int kg = 0; kg < BK / 8; kg++) {
+            C2 a0[FM], a1[FM], b0[FN], b1[FN];
+#pragma unroll
+            for (int fm = 0; fm < FM; fm++) {
+                const int base = (wm * WTM + fm * 16 + j) * LROW + kg * 8 + 2 * q;
+                a0[fm] = As[buf][base];
+                a1[fm] = As[buf][base + 1];
+            }
+#pragma unroll
+            for (int fn = 0; fn < FN; fn++) {
+                const int base = (wn * WTN + fn * 8 + (j >> 1)) * LROW + kg * 8 + 2 * q;
+                b0[fn] = Bs[buf][base];
+                b1[fn] = Bs[buf][base + 1];
+            }
+#pragma unroll
+            for (int fn = 0; fn < FN; fn++) {
+                const R g0 = t ? b0[fn].y : b0[fn].x;
+                const R g1 = t ? b0[fn].x : -b0[fn].y;
+                const R g2 = t ? b1[fn].y : b1[fn].x;
+                const R g3 = t ? b1[fn].x : -b1[fn].y;
+#pragma unroll
+                for (int fm = 0; fm < FM; fm++) {
+                    acc[fm][fn] = mfma4(a0[fm].x, g0, acc[fm][fn]);
+                    acc[fm][fn] = mfma4(a0[fm].y, g1, acc[fm][fn]);
+                    acc[fm][fn] = mfma4(a1[fm].x, g2, acc[fm][fn]);
+                    acc[fm][fn] = mfma4(a1[fm].y, g3, acc[fm][fn]);
+                }
+            }
+        }
+        if (kt + 1 < nk) swrite(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int fm = 0; fm < FM; fm++) {
+#pragma unroll
+        for (int fn = 0; fn < FN; fn++) {
+            const int64_t col = n0 + wn * WTN + fn * 8 + (j >> 1);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = m0 + wm * WTM + fm * 16 + 4 * q + r;
+                if (row < M && col < N) C[(row * g.ldc + col) * 2 + t] = acc[fm][fn][r];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// VALU kernels.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(256) void cgemm_small_kernel(GemmArgs g)
+{
+    using C2 = typename Vec<R>::c;
+    const int64_t total = g.batch * g.M * g.N;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    const C2 *__restrict__ A = static_cast<const C2 *>(g.A);
+    const C2 *__restrict__ B = static_cast<const C2 *>(g.B);
+    C2 *__restrict__ C = static_cast<C2 *>(g.C);
+    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+         idx += stride) {
+        const int64_t n = idx % g.N;
+        const int64_t rest = idx / g.N;
+        const int64_t m = rest % g.M;
+        const int64_t b = rest / g.M;
+        const C2 *a = A + b * g.sA + m * g.lda;
+        const C2 *bb = B + b * g.sB + n * g.ldb;
+        R sr = 0, si = 0;
+        for (int64_t k = 0; k < g.K; k++) {
+            const C2 x = a[k], y = bb[k];
+            sr = fma(x.x, y.x, sr);
+            sr = fma(-x.y, y.y, sr);
+            si = fma(x.x, y.y, si);
+            si = fma(x.y, y.x, si);
+        }
+        C[b * g.sC + m * g.ldc + n] = C2{sr, si};
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void cgemm_smallred_kernel(GemmArgs g)
+{
+    using C2 = typename Vec<R>::c;
+    __shared__ R red[2][256];
+    const int64_t o = blockIdx.x;
+    const int64_t n = o % g.N;
+    const int64_t rest = o / g.N;
+    const int64_t m = rest % g.M;
+    const int64_t b = rest / g.M;
+    const C2 *a = static_cast<const C2 *>(g.A) + b * g.sA + m * g.lda;
+    const C2 *bb = static_cast<const C2 *>(g.B) + b * g.sB + n * g.ldb;
+    R sr = 0, si = 0;
+    for (int64_t k = threadIdx.x; k < g.K; k += 256) {
+        const C2 x = a[k], y = bb[k];
+        sr = fma(x.x, y.x, sr);
+        sr = fma(-x.y, y.y, sr);
+        si = fma(x.x, y.y, si);
+        si = fma(x.y, y.x, si);
+    }
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        static_cast<C2 *>(g.C)[b * g.sC + m * g.ldc + n] = C2{red[0][0], red[1][0]};
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 operand preparation.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void split2(float v, __bf16 &h, __bf16 &l)
+{
+    h = (__bf16)v;
+    l = (__bf16)(v - (float)h);
+}
+
+__global__ __launch_bounds__(256) void split_bf16_kernel(
+    const float2 *__restrict__ src, int64_t rows, int64_t K, int64_t ld,
+    int64_t s, SplitOperand dst)
+{
+    const int64_t kq_n = dst.k_p / 8;
+    const int64_t total = dst.batch * dst.rows_p * kq_n;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    __bf16 *base = static_cast<__bf16 *>(dst.base);
+    for (int64_t gi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; gi < total;
+         gi += stride) {
+        const int64_t kq = gi % kq_n;
+        const int64_t rest = gi / kq_n;
+        const int64_t r = rest % dst.rows_p;
+        const int64_t b = rest / dst.rows_p;
+        bf16x8 o[6];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int64_t k = kq * 8 + e;
+            float2 x = {0.f, 0.f};
+            if (r < rows && k < K) x = src[b * s + r * ld + k];
+            __bf16 h, l;
+            split2(x.x, h, l);
+            o[0][e] = h;
+            o[1][e] = l;
+            split2(x.y, h, l);
+            o[2][e] = h;
+            o[3][e] = l;
+            split2(x.x + x.y, h, l);
+            o[4][e] = h;
+            o[5][e] = l;
+        }
+        const int64_t off = (b * dst.rows_p + r) * dst.k_p + kq * 8;
+#pragma unroll
+        for (int p = 0; p < 6; p++)
+            *reinterpret_cast<bf16x8 *>(base + p * dst.plane_elems + off) = o[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3M split-BF16 complex GEMM.
+// Stage s covers product s/nk (T1, T2, T3) and k tile s%nk: it loads the hi
+// and lo planes of that product's operands (BM x 32 and BN x 32 bf16 each)
+// and issues Xh.Yh, Xh.Yl, Xl.Yh on v_mfma_f32_16x16x32_bf16, whose lane l
+// holds 8 consecutive k (8(l/16)..8(l/16)+7) of row l%16 of A and of column
+// l%16 of B, and gets D[4(l/16)+r][l%16].  Two accumulator sets suffice:
+// x = T1 and y = T2, then (x, y) <- (T1 - T2, -T1 - T2) and T3 accumulates
+// into y, leaving x = Re C and y = Im C.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(256) void cgemm_bf16x3_kernel(
+    SplitOperand a, SplitOperand b, float2 *__restrict__ Cout, int64_t M,
+    int64_t N, int64_t ldc, int64_t sC, int a_bcast, int b_bcast)
+{
+    constexpr int KT = 32;
+    constexpr int LROW = KT + 8; // bf16 per LDS row (16-byte pad)
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    constexpr int A_CH = 2 * BM * (KT / 8) / 256; // 16-byte chunks per thread
+    constexpr int B_CH = 2 * BN * (KT / 8) / 256;
+    static_assert(WM * WN == 4 && A_CH * 256 == 2 * BM * 4 && B_CH * 256 == 2 * BN * 4, "tile");
+    __shared__ bf16x8 As[2][2 * BM * LROW / 8];
+    __shared__ bf16x8 Bs[2][2 * BN * LROW / 8];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int64_t bz = blockIdx.z;
+    const int64_t m0 = int64_t(blockIdx.x) * BM, n0 = int64_t(blockIdx.y) * BN;
+    const __bf16 *Ab = static_cast<const __bf16 *>(a.base) +
+                       (a_bcast ? 0 : bz * a.rows_p * a.k_p);
+    const __bf16 *Bb = static_cast<const __bf16 *>(b.base) +
+                       (b_bcast ? 0 : bz * b.rows_p * b.k_p);
+
+    f32x4 acc[2][FM][FN];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int jj = 0; jj < FN; jj++) acc[p][i][jj] = f32x4{0, 0, 0, 0};
+
+    bf16x8 ra[A_CH], rb[B_CH];
+    const bf16x8 zero = {};
+    const int nk = int(a.k_p / KT);
+    auto gload = [&](int s) {
+        const int64_t k0 = int64_t(s % nk) * KT;
+        const int ph = s / nk;
+#pragma unroll
+        for (int i = 0; i < A_CH; i++) {
+            const int c = tid + i * 256;
+            const int pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+            const int64_t m = m0 + r;
+            ra[i] = (m < a.rows_p)
+                        ? *reinterpret_cast<const bf16x8 *>(
+                              Ab + (2 * ph + pl) * a.plane_elems + m * a.k_p + k0 + kc * 8)
+                        : zero;
+        }
+#pragma unroll
+        for (int i = 0; i < B_CH; i++) {
+            const int c = tid + i * 256;
+            const int pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+            const int64_t n = n0 + r;
+            rb[i] = (n < b.rows_p)
+                        ? *reinterpret_cast<const bf16x8 *>(
+                              Bb + (2 * ph + pl) * b.plane_elems + n * b.k_p + k0 + kc * 8)
+                        : zero;
+        }
+    };
+    auto swrite = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < A_CH; i++) {
+            const int c = tid + i * 256;
+            const int pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+            As[buf][(pl * BM + r) * (LROW / 8) + kc] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < B_CH; i++) {
+            const int c = tid + i * 256;
+            const int pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+            Bs[buf][(pl * BN + r) * (LROW / 8) + kc] = rb[i];
+        }
+    };
+
+    const int nstages = nk * 3;
+    gload(0);
+    swrite(0);
+    __syncthreads();
+    const int q = lane >> 4, j = lane & 15;
+    for (int s = 0; s < nstages; s++) {
+        const int buf = s & 1;
+        const int ph = s / nk;
+        if (s == 2 * nk) {
+#pragma unroll
+            for (int fm = 0; fm < FM; fm++)
+#pragma unroll
+                for (int fn = 0; fn < FN; fn++) {
+                    const f32x4 t1 = acc[0][fm][fn], t2 = acc[1][fm][fn];
+                    acc[0][fm][fn] = t1 - t2;
+                    acc[1][fm][fn] = -(t1 + t2);
+                }
+        }
+        if (s + 1 < nstages) gload(s + 1);
+        bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+        for (int fm = 0; fm < FM; fm++) {
+            const int r = wm * WTM + fm * 16 + j;
+            ah[fm] = As[buf][r * (LROW / 8) + q];
+            al[fm] = As[buf][(BM + r) * (LROW / 8) + q];
+        }
+#pragma unroll
+        for (int fn = 0; fn < FN; fn++) {
+            const int r = wn * WTN + fn * 16 + j;
+            bh[fn] = Bs[buf][r * (LROW / 8) + q];
+            bl[fn] = Bs[buf][(BN + r) * (LROW / 8) + q];
+        }
+#define JH_BF16_PRODUCT(P)                                                                  \
+    _Pragma("unroll") for (int fm = 0; fm < FM; fm++)                                       \
+    {                                                                                       \
+        _Pragma("unroll") for (int fn = 0; fn < FN; fn++)                                   \
+        {                                                                                   \
+            acc[P][fm][fn] =                                                                \
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[fm], bh[fn], acc[P][fm][fn], 0, 0, 0); \
+            acc[P][fm][fn] =                                                                \
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[fm], bl[fn], acc[P][fm][fn], 0, 0, 0); \
+            acc[P][fm][fn] =                                                                \
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[fm], bh[fn], acc[P][fm][fn], 0, 0, 0); \
+        }                                                                                   \
+    }
+        if (ph == 0) {
+            JH_BF16_PRODUCT(0)
+        }
+        else {
+            JH_BF16_PRODUCT(1)
+        }
+#undef JH_BF16_PRODUCT
+        if (s + 1 < nstages) swrite(buf ^ 1);
+        __syncthreads();
+    }
+
+    float2 *C = Cout + bz * sC;
+#pragma unroll
+    for (int fm = 0; fm < FM; fm++) {
+#pragma unroll
+        for (int fn = 0; fn < FN; fn++) {
+            const int64_t col = n0 + wn * WTN + fn * 16 + j;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = m0 + wm * WTM + fm * 16 + 4 * q + r;
+                if (row < M && col < N)
+                    C[row * ldc + col] = float2{acc[0][fm][fn][r], acc[1][fm][fn][r]};
+            }
+        }
+    }
+}
+
+template <class R>
+void launch_valu(int kernel, const GemmArgs &g, hipStream_t stream)
+{
+    const int64_t total = g.batch * g.M * g.N;
+    if (kernel == kGemmSmallRed) {
+        JH_REQUIRE(total <= (int64_t(1) << 31) - 1, "Too many outputs for the reduction kernel.");
+        hipLaunchKernelGGL(cgemm_smallred_kernel<R>, dim3((unsigned)total), dim3(256), 0,
+                           stream, g);
+    }
+    else {
+        const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(cgemm_small_kernel<R>, dim3(grid), dim3(256), 0, stream, g);
+    }
+    JH_HIP(hipGetLastError());
+}
+
+} // namespace
+
+void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
+                 hipStream_t stream)
+{
+    if (g.M == 0 || g.N == 0 || g.batch == 0) return;
+    if (kernel == kGemmSmall || kernel == kGemmSmallRed) {
+        if (elem_bytes == 8)
+            launch_valu<float>(kernel, g, stream);
+        else
+            launch_valu<double>(kernel, g, stream);
+        return;
+    }
+    JH_REQUIRE(g.batch <= 65535, "Batch count exceeds 65535.");
+    if (kernel == kGemmF32Mfma) {
+        JH_REQUIRE(elem_bytes == 8, "FP32 MFMA kernel needs complex64 data.");
+        constexpr int BM = 128, BN = 128;
+        JH_REQUIRE(ceil_div(g.N, BN) <= 65535, "N too large.");
+        dim3 grid((unsigned)ceil_div(g.M, BM), (unsigned)ceil_div(g.N, BN), (unsigned)g.batch);
+        hipLaunchKernelGGL((cgemm_embed_kernel<float, BM, BN, 16, 2, 2>), grid, dim3(256), 0,
+                           stream, g);
+    }
+    else if (kernel == kGemmF64Mfma) {
+        JH_REQUIRE(elem_bytes == 16, "FP64 MFMA kernel needs complex128 data.");
+        constexpr int BM = 128, BN = 64;
+        JH_REQUIRE(ceil_div(g.N, BN) <= 65535, "N too large.");
+        dim3 grid((unsigned)ceil_div(g.M, BM), (unsigned)ceil_div(g.N, BN), (unsigned)g.batch);
+        hipLaunchKernelGGL((cgemm_embed_kernel<double, BM, BN, 8, 2, 2>), grid, dim3(256), 0,
+                           stream, g);
+    }
+    else {
+        JH_FAIL(JET_HIP_ERROR_INTERNAL, "launch_gemm: unknown kernel.");
+    }
+    JH_HIP(hipGetLastError());
+}
+
+void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
+                       int64_t s, int64_t batch, const SplitOperand &dst,
+                       hipStream_t stream)
+{
+    (void)batch;
+    const int64_t total = dst.batch * dst.rows_p * (dst.k_p / 8);
+    const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(split_bf16_kernel, dim3(grid), dim3(256), 0, stream,
+                       static_cast<const float2 *>(src), rows, k, ld, s, dst);
+    JH_HIP(hipGetLastError());
+}
+
+void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b,
+                        void *C, int64_t M, int64_t N, int64_t ldc,
+                        int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
+                        hipStream_t stream)
+{
+    constexpr int BM = 128, BN = 128;
+    JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
+    JH_REQUIRE(batch <= 65535 && ceil_div(N, BN) <= 65535, "Grid too large.");
+    dim3 grid((unsigned)ceil_div(M, BM), (unsigned)ceil_div(N, BN), (unsigned)batch);
+    hipLaunchKernelGGL((cgemm_bf16x3_kernel<BM, BN, 2, 2>), grid, dim3(256), 0, stream, a, b,
+                       static_cast<float2 *>(C), M, N, ldc, sC, a_bcast ? 1 : 0,
+                       b_bcast ? 1 : 0);
+    JH_HIP(hipGetLastError());
+}
+
+} // namespace jet_hip

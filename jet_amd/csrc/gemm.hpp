@@ -1,0 +1,71 @@
+// gemm.hpp -- complex GEMM kernels of the contraction path (gfx950).
+//
+// Every contraction reduces to the batched "NT" product
+//     C[b][m][n] = sum_k A[b][m][k] * B[b][n][k]
+// with A and B stored k-fastest (the permutation stage produces them in that
+// order).  This is the row-major NoTrans/NoTrans product of the reference
+// (TensorHelpers.hpp:47-61) with the right operand kept as [right][common]
+// so both MFMA operands are read along k.
+#pragma once
+
+#include "errors.hpp"
+
+#include <cstdint>
+
+namespace jet_hip {
+
+struct GemmArgs {
+    const void *A;
+    const void *B;
+    void *C;
+    int64_t M, N, K, batch;
+    int64_t lda, ldb, ldc; // row strides in elements
+    int64_t sA, sB, sC;    // batch strides in elements (0 = broadcast)
+};
+
+// Kernel families (also reported by jet_hip_plan_info).
+enum GemmKernel {
+    kGemmSmall = 0,     // VALU, one thread per output element
+    kGemmSmallRed = 1,  // VALU, one block per output element (long K)
+    kGemmF32Mfma = 2,   // 4M complex on v_mfma_f32_16x16x4_f32
+    kGemmF64Mfma = 3,   // 4M complex on v_mfma_f64_16x16x4_f64
+    kGemmBf16x3 = 4,    // 3M complex, split-BF16, v_mfma_f32_16x16x32_bf16
+};
+
+// Rows and k of the split-BF16 operand planes are padded to these multiples.
+constexpr int kSplitRowPad = 16;
+constexpr int kSplitKPad = 32;
+
+// Split-BF16 operand: 6 planes (re hi/lo, im hi/lo, re+im hi/lo), each
+// [batch][rows_p][k_p] bf16, plane p at offset p * plane_elems.
+struct SplitOperand {
+    void *base;
+    int64_t rows_p, k_p, batch, plane_elems;
+};
+
+inline int64_t split_plane_elems(int64_t batch, int64_t rows, int64_t k)
+{
+    const int64_t rp = ceil_div(rows, kSplitRowPad) * kSplitRowPad;
+    const int64_t kp = ceil_div(k, kSplitKPad) * kSplitKPad;
+    return batch * rp * kp;
+}
+
+// Which kernel the plan uses for a given shape and precision mode
+// (0 = fp32/complex64, 1 = bf16x3/complex64, 2 = fp64/complex128).
+int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
+                       int64_t batch);
+
+#if defined(__HIPCC__)
+void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
+                 hipStream_t stream);
+// src: complex64 [batch][rows][k] with row stride ld and batch stride s.
+void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
+                       int64_t s, int64_t batch, const SplitOperand &dst,
+                       hipStream_t stream);
+void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b,
+                        void *C, int64_t M, int64_t N, int64_t ldc,
+                        int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
+                        hipStream_t stream);
+#endif
+
+} // namespace jet_hip

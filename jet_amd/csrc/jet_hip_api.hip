@@ -1,0 +1,475 @@
+// jet_hip_api.hip -- the extern "C" boundary of libjet_hip.so (include/jet_hip.h).
+//
+// Every entry point catches all C++ exceptions and turns them into a status
+// code plus a thread-local message (Jet::Exception cannot cross a C ABI).
+#include "common.hpp"
+#include "contract_plan.hpp"
+#include "gemm.hpp"
+#include "misc.hpp"
+#include "permute_plan.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace jet_hip;
+
+struct jet_hip_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct jet_hip_plan_s {
+    jet_hip_ctx_s *ctx = nullptr;
+    ContractionDesc d;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string &msg)
+{
+    g_last_error = msg;
+    return status;
+}
+
+template <class F> int guarded(F &&f)
+{
+    try {
+        f();
+        return JET_HIP_SUCCESS;
+    }
+    catch (const Error &e) {
+        return fail(e.status, e.what());
+    }
+    catch (const std::bad_alloc &) {
+        return fail(JET_HIP_ERROR_OUT_OF_MEMORY, "Error in Jet: host allocation failed.");
+    }
+    catch (const std::exception &e) {
+        return fail(JET_HIP_ERROR_INTERNAL, std::string("Error in Jet: ") + e.what());
+    }
+    catch (...) {
+        return fail(JET_HIP_ERROR_INTERNAL, "Error in Jet: unknown exception.");
+    }
+}
+
+int elem_bytes_of(jet_hip_dtype_t dt)
+{
+    if (dt == JET_HIP_C64) return 8;
+    if (dt == JET_HIP_C128) return 16;
+    JH_FAIL(JET_HIP_ERROR_INVALID_VALUE, "Unsupported data type.");
+}
+
+hipStream_t pick_stream(jet_hip_ctx_s *ctx, void *stream)
+{
+    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+void set_device(jet_hip_ctx_s *ctx)
+{
+    JH_REQUIRE(ctx != nullptr, "Null context.");
+    JH_HIP(hipSetDevice(ctx->device));
+}
+
+int mode_of(jet_hip_dtype_t dt, jet_hip_precision_t p)
+{
+    switch (p) {
+    case JET_HIP_PREC_DEFAULT:
+        return dt == JET_HIP_C64 ? 0 : 2;
+    case JET_HIP_PREC_FP32:
+        return 0;
+    case JET_HIP_PREC_BF16X3:
+        return 1;
+    case JET_HIP_PREC_FP64:
+        return 2;
+    }
+    JH_FAIL(JET_HIP_ERROR_INVALID_VALUE, "Unknown precision.");
+}
+
+// Row-major permutation plan from mode lists (layout converts col-major).
+PermutePlan permute_plan_from_modes(int layout, int rank, const int32_t *in_modes,
+                                    const int64_t *extents, const int32_t *out_modes)
+{
+    std::vector<int32_t> im(in_modes, in_modes + rank), om(out_modes, out_modes + rank);
+    std::vector<int64_t> ex(extents, extents + rank);
+    if (layout == JET_HIP_COL_MAJOR) {
+        std::reverse(im.begin(), im.end());
+        std::reverse(om.begin(), om.end());
+        std::reverse(ex.begin(), ex.end());
+    }
+    std::vector<int32_t> perm(rank);
+    for (int i = 0; i < rank; i++) {
+        int pos = -1;
+        for (int j = 0; j < rank; j++)
+            if (im[j] == om[i]) {
+                JH_REQUIRE(pos < 0, "Duplicate existing indices found. Please ensure indices are unique.");
+                pos = j;
+            }
+        JH_REQUIRE(pos >= 0, "New indices are an invalid permutation of the existing indices");
+        perm[i] = pos;
+    }
+    return make_permute_plan(rank, ex.data(), perm.data());
+}
+
+void run_contraction(const ContractionDesc &d, const void *a, const void *b, void *c,
+                     char *ws, hipStream_t s)
+{
+    const int eb = d.elem_bytes;
+    const void *At = a;
+    const void *Bt = b;
+    if (d.need_a) {
+        launch_permute(d.pa, eb, a, ws + d.off_at, s);
+        At = ws + d.off_at;
+    }
+    if (d.need_b) {
+        launch_permute(d.pb, eb, b, ws + d.off_bt, s);
+        Bt = ws + d.off_bt;
+    }
+    void *Ct = d.need_c ? static_cast<void *>(ws + d.off_ct) : c;
+    if (d.gemm_kernel == kGemmBf16x3) {
+        SplitOperand sa{ws + d.off_sa, ceil_div(d.M, kSplitRowPad) * kSplitRowPad,
+                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_a_plane};
+        SplitOperand sb{ws + d.off_sb, ceil_div(d.N, kSplitRowPad) * kSplitRowPad,
+                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_b_plane};
+        launch_split_bf16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
+        launch_split_bf16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
+        launch_gemm_bf16x3(sa, sb, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
+    }
+    else {
+        GemmArgs g;
+        g.A = At;
+        g.B = Bt;
+        g.C = Ct;
+        g.M = d.M;
+        g.N = d.N;
+        g.K = d.K;
+        g.batch = d.batch;
+        g.lda = d.K;
+        g.ldb = d.K;
+        g.ldc = d.N;
+        g.sA = d.M * d.K;
+        g.sB = d.N * d.K;
+        g.sC = d.M * d.N;
+        launch_gemm(d.gemm_kernel, eb, g, s);
+    }
+    if (d.need_c) launch_permute(d.pc, eb, Ct, c, s);
+}
+
+} // namespace
+
+extern "C" {
+
+int jet_hip_get_version(int *major, int *minor, int *patch)
+{
+    if (major) *major = JET_HIP_VERSION_MAJOR;
+    if (minor) *minor = JET_HIP_VERSION_MINOR;
+    if (patch) *patch = JET_HIP_VERSION_PATCH;
+    return JET_HIP_SUCCESS;
+}
+
+const char *jet_hip_last_error(void) { return g_last_error.c_str(); }
+
+int jet_hip_ctx_create(int device, jet_hip_ctx_t *ctx)
+{
+    return guarded([&] {
+        JH_REQUIRE(ctx != nullptr, "Null output pointer.");
+        int count = 0;
+        JH_HIP(hipGetDeviceCount(&count));
+        JH_REQUIRE(device >= 0 && device < count, "Invalid device ordinal.");
+        JH_HIP(hipSetDevice(device));
+        auto *c = new jet_hip_ctx_s;
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            JH_HIP(e);
+        }
+        // Keep freed stream-ordered allocations cached for reuse.
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t threshold = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+        }
+        *ctx = c;
+    });
+}
+
+int jet_hip_ctx_destroy(jet_hip_ctx_t ctx)
+{
+    return guarded([&] {
+        if (!ctx) return;
+        set_device(ctx);
+        JH_HIP(hipStreamSynchronize(ctx->stream));
+        JH_HIP(hipStreamDestroy(ctx->stream));
+        delete ctx;
+    });
+}
+
+int jet_hip_ctx_device(jet_hip_ctx_t ctx, int *device)
+{
+    return guarded([&] {
+        JH_REQUIRE(ctx && device, "Null argument.");
+        *device = ctx->device;
+    });
+}
+
+int jet_hip_ctx_stream(jet_hip_ctx_t ctx, void **stream)
+{
+    return guarded([&] {
+        JH_REQUIRE(ctx && stream, "Null argument.");
+        *stream = ctx->stream;
+    });
+}
+
+int jet_hip_malloc(jet_hip_ctx_t ctx, size_t bytes, void **ptr)
+{
+    return guarded([&] {
+        set_device(ctx);
+        JH_REQUIRE(ptr != nullptr, "Null output pointer.");
+        *ptr = nullptr;
+        if (bytes == 0) bytes = 1;
+        // Stream-ordered pool allocation on the context stream, made visible
+        // to every stream before returning.
+        JH_HIP(hipMallocAsync(ptr, bytes, ctx->stream));
+        JH_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int jet_hip_free(jet_hip_ctx_t ctx, void *ptr)
+{
+    return guarded([&] {
+        set_device(ctx);
+        // Ordered after all work queued on the context stream; callers that
+        // used other streams synchronize them first.
+        if (ptr) JH_HIP(hipFreeAsync(ptr, ctx->stream));
+    });
+}
+
+int jet_hip_memcpy(jet_hip_ctx_t ctx, void *dst, const void *src, size_t bytes, int kind,
+                   void *stream, int async)
+{
+    return guarded([&] {
+        set_device(ctx);
+        if (bytes == 0) return;
+        hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost
+                                      : hipMemcpyDeviceToDevice;
+        hipStream_t s = pick_stream(ctx, stream);
+        JH_HIP(hipMemcpyAsync(dst, src, bytes, k, s));
+        if (!async) JH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int jet_hip_memset(jet_hip_ctx_t ctx, void *dst, int value, size_t bytes, void *stream)
+{
+    return guarded([&] {
+        set_device(ctx);
+        if (bytes) JH_HIP(hipMemsetAsync(dst, value, bytes, pick_stream(ctx, stream)));
+    });
+}
+
+int jet_hip_stream_synchronize(jet_hip_ctx_t ctx, void *stream)
+{
+    return guarded([&] {
+        set_device(ctx);
+        JH_HIP(hipStreamSynchronize(pick_stream(ctx, stream)));
+    });
+}
+
+int jet_hip_device_synchronize(jet_hip_ctx_t ctx)
+{
+    return guarded([&] {
+        set_device(ctx);
+        JH_HIP(hipDeviceSynchronize());
+    });
+}
+
+int jet_hip_permute(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, jet_hip_layout_t layout, int rank,
+                    const int32_t *in_modes, const int64_t *extents, const int32_t *out_modes,
+                    const void *in, void *out, void *stream)
+{
+    return guarded([&] {
+        set_device(ctx);
+        const int eb = elem_bytes_of(dtype);
+        JH_REQUIRE(rank >= 1, "Number of indices cannot be zero.");
+        JH_REQUIRE(rank <= kMaxRank, "Permutation rank exceeds 64.");
+        PermutePlan p = permute_plan_from_modes(layout, rank, in_modes, extents, out_modes);
+        launch_permute(p, eb, in, out, pick_stream(ctx, stream));
+    });
+}
+
+int jet_hip_contraction_plan_create(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
+                                    jet_hip_precision_t precision, jet_hip_layout_t layout,
+                                    int rank_a, const int32_t *modes_a, const int64_t *extents_a,
+                                    int rank_b, const int32_t *modes_b, const int64_t *extents_b,
+                                    int rank_c, const int32_t *modes_c, jet_hip_plan_t *plan)
+{
+    return guarded([&] {
+        JH_REQUIRE(ctx && plan, "Null argument.");
+        const int eb = elem_bytes_of(dtype);
+        auto *p = new jet_hip_plan_s;
+        try {
+            p->ctx = ctx;
+            p->d = plan_contraction(eb, mode_of(dtype, precision), layout, rank_a, modes_a,
+                                    extents_a, rank_b, modes_b, extents_b, rank_c, modes_c);
+        }
+        catch (...) {
+            delete p;
+            throw;
+        }
+        *plan = p;
+    });
+}
+
+int jet_hip_plan_workspace_size(jet_hip_plan_t plan, size_t *bytes)
+{
+    return guarded([&] {
+        JH_REQUIRE(plan && bytes, "Null argument.");
+        *bytes = plan->d.workspace;
+    });
+}
+
+int jet_hip_plan_info(jet_hip_plan_t plan, double *flops, int64_t *m, int64_t *n, int64_t *k,
+                      int64_t *batch, int *gemm_kernel)
+{
+    return guarded([&] {
+        JH_REQUIRE(plan, "Null plan.");
+        if (flops) *flops = plan->d.flops;
+        if (m) *m = plan->d.M;
+        if (n) *n = plan->d.N;
+        if (k) *k = plan->d.K;
+        if (batch) *batch = plan->d.batch;
+        if (gemm_kernel) *gemm_kernel = plan->d.gemm_kernel;
+    });
+}
+
+int jet_hip_plan_destroy(jet_hip_plan_t plan)
+{
+    delete plan;
+    return JET_HIP_SUCCESS;
+}
+
+int jet_hip_contract(jet_hip_plan_t plan, const void *a, const void *b, void *c, void *workspace,
+                     size_t workspace_bytes, void *stream)
+{
+    return guarded([&] {
+        JH_REQUIRE(plan != nullptr, "Null plan.");
+        jet_hip_ctx_s *ctx = plan->ctx;
+        set_device(ctx);
+        const ContractionDesc &d = plan->d;
+        JH_REQUIRE(a && b && c, "Null tensor data.");
+        hipStream_t s = pick_stream(ctx, stream);
+        char *ws = static_cast<char *>(workspace);
+        bool pooled = false;
+        if (d.workspace > 0 && ws == nullptr) {
+            void *p = nullptr;
+            JH_HIP(hipMallocAsync(&p, d.workspace, s));
+            ws = static_cast<char *>(p);
+            pooled = true;
+        }
+        else if (d.workspace > 0) {
+            JH_REQUIRE(workspace_bytes >= d.workspace, "Workspace is smaller than the plan needs.");
+        }
+        try {
+            run_contraction(d, a, b, c, ws, s);
+        }
+        catch (...) {
+            if (pooled) (void)hipFreeAsync(ws, s);
+            throw;
+        }
+        if (pooled) JH_HIP(hipFreeAsync(ws, s));
+    });
+}
+
+int jet_hip_add(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, jet_hip_layout_t layout, int rank,
+                const int32_t *modes_a, const int64_t *extents_a, const int32_t *modes_b,
+                const void *a, const void *b, void *c, void *stream)
+{
+    return guarded([&] {
+        set_device(ctx);
+        const int eb = elem_bytes_of(dtype);
+        hipStream_t s = pick_stream(ctx, stream);
+        int64_t n = 1;
+        for (int i = 0; i < rank; i++) n *= extents_a[i];
+        if (rank == 0) {
+            launch_add(eb, a, b, c, 1, s);
+            return;
+        }
+        // b has modes_b; its extents follow from a's.
+        std::vector<int64_t> eb_ext(rank);
+        for (int i = 0; i < rank; i++) {
+            int pos = -1;
+            for (int j = 0; j < rank; j++)
+                if (modes_a[j] == modes_b[i]) pos = j;
+            JH_REQUIRE(pos >= 0, "Tensor addition with disjoint indices is not supported.");
+            eb_ext[i] = extents_a[pos];
+        }
+        PermutePlan p = permute_plan_from_modes(layout, rank, modes_b, eb_ext.data(), modes_a);
+        if (p.kind == PermutePlan::kCopy) {
+            launch_add(eb, a, b, c, n, s);
+            return;
+        }
+        void *tmp = nullptr;
+        JH_HIP(hipMallocAsync(&tmp, (size_t)eb * n, s));
+        try {
+            launch_permute(p, eb, b, tmp, s);
+            launch_add(eb, a, tmp, c, n, s);
+        }
+        catch (...) {
+            (void)hipFreeAsync(tmp, s);
+            throw;
+        }
+        JH_HIP(hipFreeAsync(tmp, s));
+    });
+}
+
+int jet_hip_slice(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, jet_hip_layout_t layout, int rank,
+                  const int64_t *extents, int axis, int64_t value, const void *in, void *out,
+                  void *stream)
+{
+    return guarded([&] {
+        set_device(ctx);
+        const int eb = elem_bytes_of(dtype);
+        JH_REQUIRE(rank >= 1 && axis >= 0 && axis < rank, "Sliced index does not exist.");
+        std::vector<int64_t> ex(extents, extents + rank);
+        int ax = axis;
+        if (layout == JET_HIP_COL_MAJOR) {
+            std::reverse(ex.begin(), ex.end());
+            ax = rank - 1 - axis;
+        }
+        JH_REQUIRE(value >= 0 && value < ex[ax], "Slice value is out of range.");
+        int64_t pre = 1, post = 1;
+        for (int i = 0; i < ax; i++) pre *= ex[i];
+        for (int i = ax + 1; i < rank; i++) post *= ex[i];
+        const size_t width = (size_t)post * eb;
+        const char *src = static_cast<const char *>(in) + (size_t)value * width;
+        JH_HIP(hipMemcpy2DAsync(out, width, src, width * (size_t)ex[ax], width, (size_t)pre,
+                                hipMemcpyDeviceToDevice, pick_stream(ctx, stream)));
+    });
+}
+
+int jet_hip_fill_random(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, size_t n, uint64_t seed,
+                        int distribution, void *out, void *stream)
+{
+    return guarded([&] {
+        set_device(ctx);
+        launch_fill(elem_bytes_of(dtype), out, (int64_t)n, seed, distribution,
+                    pick_stream(ctx, stream));
+    });
+}
+
+int jet_hip_reduce_compensated(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, size_t n, const void *x,
+                               double *out, void *stream)
+{
+    return guarded([&] {
+        set_device(ctx);
+        JH_REQUIRE(out != nullptr, "Null output.");
+        launch_reduce(elem_bytes_of(dtype), x, (int64_t)n, out, pick_stream(ctx, stream));
+    });
+}
+
+} // extern "C"

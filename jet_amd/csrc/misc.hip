@@ -1,0 +1,135 @@
+// misc.hip -- elementwise add, seeded fill and the FP64 compensated
+// reduction (gfx950).
+#include "common.hpp"
+#include "misc.hpp"
+
+#include <algorithm>
+
+namespace jet_hip {
+
+namespace {
+
+template <class C2>
+__global__ __launch_bounds__(256) void add_kernel(const C2 *__restrict__ a,
+                                                  const C2 *__restrict__ b,
+                                                  C2 *__restrict__ c, int64_t n)
+{
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const C2 x = a[i], y = b[i];
+        c[i] = C2{x.x + y.x, x.y + y.y};
+    }
+}
+
+template <class C2>
+__global__ __launch_bounds__(256) void fill_kernel(C2 *__restrict__ out, int64_t n,
+                                                   uint64_t seed, int dist)
+{
+    using R = decltype(C2{}.x);
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double u0 = counter_uniform(seed, uint64_t(2 * i));
+        const double u1 = counter_uniform(seed, uint64_t(2 * i + 1));
+        double re, im;
+        counter_transform(dist, u0, u1, &re, &im);
+        out[i] = C2{R(re), R(im)};
+    }
+}
+
+// Error-free sum (Knuth TwoSum): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+template <class C2>
+__global__ __launch_bounds__(1024) void reduce_kernel(const C2 *__restrict__ x, int64_t n,
+                                                      double *__restrict__ out)
+{
+    __shared__ double hs[2][1024];
+    __shared__ double ls[2][1024];
+    // Each thread: double-double (hi, lo) accumulation of its strided subset.
+    double h[2] = {0.0, 0.0}, l[2] = {0.0, 0.0};
+    for (int64_t i = threadIdx.x; i < n; i += 1024) {
+        const C2 v = x[i];
+        const double vv[2] = {double(v.x), double(v.y)};
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            double s, e;
+            two_sum(h[p], vv[p], s, e);
+            h[p] = s;
+            l[p] += e;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        hs[p][threadIdx.x] = h[p];
+        ls[p][threadIdx.x] = l[p];
+    }
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) {
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                double sum, e;
+                two_sum(hs[p][threadIdx.x], hs[p][threadIdx.x + s], sum, e);
+                hs[p][threadIdx.x] = sum;
+                ls[p][threadIdx.x] += ls[p][threadIdx.x + s] + e;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = hs[0][0] + ls[0][0];
+        out[1] = hs[1][0] + ls[1][0];
+    }
+}
+
+int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
+
+} // namespace
+
+void launch_add(int elem_bytes, const void *a, const void *b, void *c, int64_t n,
+                hipStream_t stream)
+{
+    if (n == 0) return;
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(add_kernel<float2>, dim3(grid_for(n)), dim3(256), 0, stream,
+                           static_cast<const float2 *>(a), static_cast<const float2 *>(b),
+                           static_cast<float2 *>(c), n);
+    else
+        hipLaunchKernelGGL(add_kernel<double2>, dim3(grid_for(n)), dim3(256), 0, stream,
+                           static_cast<const double2 *>(a), static_cast<const double2 *>(b),
+                           static_cast<double2 *>(c), n);
+    JH_HIP(hipGetLastError());
+}
+
+void launch_fill(int elem_bytes, void *out, int64_t n, uint64_t seed, int dist,
+                 hipStream_t stream)
+{
+    if (n == 0) return;
+    JH_REQUIRE(dist >= 0 && dist <= 2, "Unknown distribution.");
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(fill_kernel<float2>, dim3(grid_for(n)), dim3(256), 0, stream,
+                           static_cast<float2 *>(out), n, seed, dist);
+    else
+        hipLaunchKernelGGL(fill_kernel<double2>, dim3(grid_for(n)), dim3(256), 0, stream,
+                           static_cast<double2 *>(out), n, seed, dist);
+    JH_HIP(hipGetLastError());
+}
+
+void launch_reduce(int elem_bytes, const void *x, int64_t n, double *out,
+                   hipStream_t stream)
+{
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(reduce_kernel<float2>, dim3(1), dim3(1024), 0, stream,
+                           static_cast<const float2 *>(x), n, out);
+    else
+        hipLaunchKernelGGL(reduce_kernel<double2>, dim3(1), dim3(1024), 0, stream,
+                           static_cast<const double2 *>(x), n, out);
+    JH_HIP(hipGetLastError());
+}
+
+} // namespace jet_hip

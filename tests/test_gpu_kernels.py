@@ -1,0 +1,300 @@
+"""Parity of the HIP kernels with the oracle, through the C ABI.
+
+Bit-exact for permutations, slices, copies and adds; FP contractions within
+the tolerances stated per test (normwise relative error against the FP64
+oracle): 1e-5 for complex64 on FP32 MFMA and on split-BF16, 1e-12 for
+complex128 on FP64 MFMA.
+"""
+import string
+
+import numpy as np
+import pytest
+
+import oracle
+from jet_amd import _native as nat
+from jet_amd import Permuter, Tensor
+
+from hiputil import contract, permute, rand_c, rel_err, say
+
+pytestmark = pytest.mark.gpu
+
+TOL = {nat.PREC_FP32: 1e-5, nat.PREC_BF16X3: 1e-5, nat.PREC_FP64: 1e-12,
+       nat.PREC_DEFAULT: 1e-5}
+
+
+# --------------------------------------------------------------------------
+# permutations
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_permuter_reference_fixtures(ctx, golden_permuter, dtype):
+    p = Permuter(ctx)
+    for case in golden_permuter["cases"]:
+        say(f"permuter fixture {case['shape']} {case['old_labels']} {np.dtype(dtype)}")
+        data = np.array([complex(a, b) for a, b in golden_permuter["arrays"][case["input"]]],
+                        dtype=dtype)
+        exp = np.array([complex(a, b) for a, b in case["expected"]], dtype=dtype)
+        got = p.transpose(data, case["shape"], case["old_labels"], case["new_labels"])
+        assert np.array_equal(got, exp), case
+        out = np.zeros_like(data)
+        p.transpose(data, case["shape"], case["old_labels"], case["new_labels"], out)
+        assert np.array_equal(out, exp)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_permute_random_shapes(ctx, dtype):
+    rng = np.random.default_rng(21)
+    for t in range(120):
+        r = int(rng.integers(1, 20))
+        shape = [int(rng.choice([1, 2, 2, 2, 2, 4, 8, 3, 5])) for _ in range(r)]
+        while np.prod(shape) > 2 ** 18:
+            shape.pop()
+        perm = [int(p) for p in rng.permutation(len(shape))]
+        n = int(np.prod(shape))
+        x = rand_c(rng, n, dtype)
+        say(f"permute {shape} {perm}")
+        got = permute(ctx, x, shape, list(range(len(shape))), perm)
+        assert np.array_equal(got, oracle.permute_default(x, shape, perm)), (shape, perm)
+
+
+def test_permute_column_major_is_reversed_row_major(ctx):
+    rng = np.random.default_rng(4)
+    shape = [2, 3, 4, 5]
+    x = rand_c(rng, 120, np.complex64)
+    modes_in, modes_out = [10, 11, 12, 13], [12, 10, 13, 11]
+    say("permute col-major")
+    got = permute(ctx, x, shape, modes_in, modes_out, layout=nat.COL_MAJOR)
+    # column-major (i0 fastest) == row-major with modes and shape reversed
+    rm_in = modes_in[::-1]
+    want = oracle.permute_default(x, shape[::-1], [rm_in.index(m) for m in modes_out[::-1]])
+    assert np.array_equal(got, want)
+
+
+def test_permute_large_bond2(ctx):
+    rng = np.random.default_rng(8)
+    r = 24
+    perm = [int(p) for p in rng.permutation(r)]
+    x = rand_c(rng, 2 ** r, np.complex64)
+    say("permute rank-24 bond-2")
+    got = permute(ctx, x, [2] * r, list(range(r)), perm)
+    want = np.ascontiguousarray(np.transpose(x.reshape([2] * r), perm)).reshape(-1)
+    assert np.array_equal(got, want)
+
+
+# --------------------------------------------------------------------------
+# contractions
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_contract_reference_literals(ctx, dtype):
+    say("contract literals")
+    # Test_Tensor.cpp:477-563
+    t1 = Tensor(["a", "b"], [2, 2], [0, 1, 1, 0], dtype=dtype)
+    t2 = Tensor(["b"], [2], [1, 0], dtype=dtype)
+    assert Tensor.ContractTensors(t1, t2) == Tensor(["a"], [2], [0, 1], dtype=dtype)
+    t1 = Tensor(["a"], [2], [0, 1], dtype=dtype)
+    t2 = Tensor(["a", "b"], [2, 2], [0, 1, 2, 3], dtype=dtype)
+    assert t1.contract_with_tensor(t2) == Tensor(["b"], [2], [2, 3], dtype=dtype)
+    t1 = Tensor(["a", "b", "c"], [2, 3, 4], np.full(24, 0.5 + 0.25j), dtype=dtype)
+    t2 = Tensor(["b", "c", "d"], [3, 4, 2], np.full(24, 0.5 + 0.25j), dtype=dtype)
+    assert t1.contract_with_tensor(t2) == Tensor(["a", "d"], [2, 2], [2.25 + 3j] * 4, dtype=dtype)
+    t1 = Tensor(["a", "b"], [2, 2], np.ones(4), dtype=dtype)
+    t2 = Tensor(["a", "b"], [2, 2], np.ones(4), dtype=dtype)
+    t3 = Tensor.ContractTensors(t1, t2)
+    assert t3 == Tensor([], [], [4], dtype=dtype) and t3.is_scalar()
+
+
+def test_contract_cuda_layout_mapping(ctx):
+    """Test_CudaTensor.cpp:561-625: column-major A(a,b,c) . B(c,b,d) equals the
+    row-major contraction of the reversed tensors, with C's modes reversed."""
+    rng = np.random.default_rng(7)
+    a = rand_c(rng, 30, np.complex64)
+    b = rand_c(rng, 60, np.complex64)
+    say("contract col-major mapping")
+    # column-major: modes (a,b,c) extents (2,3,5) and (c,b,d) extents (5,3,4); C = (a,d)
+    got, _ = contract(ctx, a, [0, 1, 2], [2, 3, 5], b, [2, 1, 3], [5, 3, 4], [0, 3],
+                      layout=nat.COL_MAJOR)
+    lc, sc, want = oracle.contract(["c", "b", "a"], [5, 3, 2], a, ["d", "b", "c"], [4, 3, 5], b)
+    assert lc == ["d", "a"]
+    # column-major (a,d) data == row-major (d,a) data
+    assert rel_err(got, want) < 1e-6
+
+
+def _random_problem(rng, max_rank=6, dims=(1, 2, 3, 4)):
+    pool = list(string.ascii_letters[:12])
+    na, nb = int(rng.integers(0, max_rank + 1)), int(rng.integers(0, max_rank + 1))
+    la = list(rng.choice(pool, size=na, replace=False))
+    lb = list(rng.choice(pool, size=nb, replace=False))
+    d = {k: int(rng.choice(dims)) for k in pool}
+    return la, [d[x] for x in la], lb, [d[x] for x in lb]
+
+
+@pytest.mark.parametrize("prec,dtype", [(nat.PREC_FP32, np.complex64),
+                                        (nat.PREC_BF16X3, np.complex64),
+                                        (nat.PREC_FP64, np.complex128)])
+def test_contract_random_small(ctx, prec, dtype):
+    rng = np.random.default_rng(100 + prec)
+    for t in range(80):
+        la, sa, lb, sb = _random_problem(rng)
+        a = rand_c(rng, int(np.prod(sa)), dtype)
+        b = rand_c(rng, int(np.prod(sb)), dtype)
+        lc, sc, want = oracle.contract(la, sa, a, lb, sb, b)
+        ids = {s: i for i, s in enumerate(string.ascii_letters)}
+        say(f"contract {la}{sa} x {lb}{sb} prec={prec}")
+        got, info = contract(ctx, a, [ids[x] for x in la], sa, b, [ids[x] for x in lb], sb,
+                             [ids[x] for x in lc], precision=prec, workspace=(t % 2 == 0))
+        assert rel_err(got, want) < TOL[prec], (la, sa, lb, sb, info)
+
+
+@pytest.mark.parametrize("prec,dtype,shape", [
+    (nat.PREC_FP32, np.complex64, (256, 256, 256)),
+    (nat.PREC_FP32, np.complex64, (300, 70, 170)),
+    (nat.PREC_FP32, np.complex64, (33, 17, 16)),
+    (nat.PREC_BF16X3, np.complex64, (512, 512, 512)),
+    (nat.PREC_BF16X3, np.complex64, (300, 200, 170)),
+    (nat.PREC_FP64, np.complex128, (256, 128, 96)),
+    (nat.PREC_FP64, np.complex128, (130, 70, 50)),
+    (nat.PREC_FP32, np.complex64, (8, 4, 4000)),
+    (nat.PREC_FP64, np.complex128, (3, 5, 3000)),
+])
+def test_gemm_kernels(ctx, prec, dtype, shape):
+    """Matrix-shaped contractions that land on each GEMM kernel."""
+    M, N, K = shape
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    a = rand_c(rng, M * K, dtype)
+    b = rand_c(rng, K * N, dtype)
+    # A(m,k) . B(k,n): B is stored [k][n] like the reference's Bt.
+    say(f"gemm {shape} prec={prec}")
+    got, info = contract(ctx, a, [0, 1], [M, K], b, [1, 2], [K, N], [0, 2], precision=prec)
+    want = oracle.gemm(a.reshape(M, K), b.reshape(K, N)).reshape(-1)
+    err = rel_err(got, want)
+    say(f"  kernel={info['kernel']} err={err:.3e}")
+    assert err < TOL[prec], (shape, info, err)
+    if prec == nat.PREC_BF16X3 and min(shape) >= 128:
+        assert info["kernel"] == "bf16x3_mfma"
+
+
+def test_batch_modes(ctx):
+    """Modes present in A, B and C are batch modes (used by the slice loop)."""
+    rng = np.random.default_rng(12)
+    # A(s, m, k), B(k, s, n) -> C(s, m, n)
+    S, M, K, N = 4, 40, 24, 20
+    a = rand_c(rng, S * M * K, np.complex64)
+    b = rand_c(rng, K * S * N, np.complex64)
+    say("batch modes")
+    got, info = contract(ctx, a, [0, 1, 2], [S, M, K], b, [2, 0, 3], [K, S, N], [0, 1, 3])
+    want = np.einsum("smk,ksn->smn", a.reshape(S, M, K).astype(np.complex128),
+                     b.reshape(K, S, N).astype(np.complex128)).reshape(-1)
+    assert info["batch"] == S
+    assert rel_err(got, want) < 1e-5
+
+
+def test_contract_rejects_bad_modes(ctx):
+    with pytest.raises(nat.JetHipError, match="neither contracted nor kept"):
+        contract(ctx, np.ones(4, np.complex64), [0, 1], [2, 2], np.ones(2, np.complex64), [1],
+                 [2], [])
+    with pytest.raises(nat.JetHipError, match="different extents"):
+        contract(ctx, np.ones(4, np.complex64), [0, 1], [2, 2], np.ones(3, np.complex64), [1],
+                 [3], [0])
+
+
+# --------------------------------------------------------------------------
+# add / slice / fill / reduce
+# --------------------------------------------------------------------------
+
+def test_add_tensors_reference_values(ctx):
+    say("add")
+    # Test_Tensor.cpp AddTensors sections and Test_CudaTensor.cpp:627-678
+    lhs = Tensor([], [], [1])
+    rhs = Tensor([], [], [2 + 4j])
+    assert Tensor.AddTensors(lhs, rhs) == Tensor([], [], [3 + 4j])
+    lhs = Tensor(["i"], [3], [1, 2j, 3])
+    rhs = Tensor(["i"], [3], [1, 2j, 3j])
+    assert lhs.add_tensor(rhs) == Tensor(["i"], [3], [2, 4j, 3 + 3j])
+    lhs = Tensor(["i", "j"], [2, 2], [1, 2, 3, 4])
+    rhs = Tensor(["j", "i"], [2, 2], [1j, 2j, 3j, 4j])
+    # row-major: rhs[j][i]; want[i][j] = lhs[i][j] + rhs[j][i]
+    assert Tensor.AddTensors(lhs, rhs) == Tensor(["i", "j"], [2, 2], [1 + 1j, 2 + 3j, 3 + 2j, 4 + 4j])
+    zero = Tensor()
+    assert Tensor.AddTensors(zero, lhs) == lhs
+
+
+def test_add_column_major_swapped(ctx):
+    """Test_CudaTensor.cpp:665-677 through the column-major ABI."""
+    say("add col-major")
+    a = np.array([1, 2, 3, 4], np.complex64)
+    b = np.array([1j, 2j, 3j, 4j], np.complex64)
+    from hiputil import to_device, from_device
+    da, db = to_device(ctx, a), to_device(ctx, b)
+    dc = nat.DeviceBuffer(32, ctx)
+    nat.check(nat.lib().jet_hip_add(ctx.handle, nat.C64, nat.COL_MAJOR, 2, nat.i32([0, 1]),
+                                    nat.i64([2, 2]), nat.i32([1, 0]), da.ptr, db.ptr, dc.ptr,
+                                    None))
+    assert np.array_equal(from_device(dc, 4, np.complex64), [1 + 1j, 2 + 3j, 3 + 2j, 4 + 4j])
+
+
+def test_slice_index_reference_values(ctx):
+    say("slice")
+    # Test_Tensor.cpp:579-599
+    t = Tensor(["x", "y"], [2, 3], [1, 2, 3, 4, 5, 6])
+    assert t.slice_index("x", 0) == Tensor(["y"], [3], [1, 2, 3])
+    assert Tensor.SliceIndex(t, "x", 1) == Tensor(["y"], [3], [4, 5, 6])
+    assert t.slice_index("y", 0) == Tensor(["x"], [2], [1, 4])
+    assert t.slice_index("y", 1) == Tensor(["x"], [2], [2, 5])
+    assert t.slice_index("y", 2) == Tensor(["x"], [2], [3, 6])
+    rng = np.random.default_rng(3)
+    shape = [3, 2, 4, 5]
+    x = rand_c(rng, 120, np.complex128)
+    t = Tensor(["a", "b", "c", "d"], shape, x, dtype=np.complex128)
+    for ax, lab in enumerate("abcd"):
+        for v in range(shape[ax]):
+            l, s, want = oracle.slice_index("abcd", shape, x, lab, v)
+            got = t.slice_index(lab, v)
+            assert got.indices == l and got.shape == s
+            assert np.array_equal(got.get_data(), want)
+
+
+def _counter_uniform(seed, counter):
+    """numpy restatement of the documented counter RNG (splitmix64)."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+    def sm(z):
+        z = (z + np.uint64(0x9E3779B97F4A7C15)) & M
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & M
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & M
+        return z ^ (z >> np.uint64(31))
+
+    with np.errstate(over="ignore"):
+        x = sm(np.uint64(seed) ^ sm(counter.astype(np.uint64)))
+    return (x >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def test_fill_random_is_counter_based(ctx):
+    say("fill")
+    n = 5000
+    from hiputil import from_device
+    buf = nat.DeviceBuffer(n * 16, ctx)
+    nat.check(nat.lib().jet_hip_fill_random(ctx.handle, nat.C128, n, 2026, 1, buf.ptr, None))
+    got = from_device(buf, n, np.complex128)
+    i = np.arange(n, dtype=np.uint64)
+    want = (2 * _counter_uniform(2026, 2 * i) - 1) + 1j * (2 * _counter_uniform(2026, 2 * i + 1) - 1)
+    assert np.array_equal(got, want)
+    nat.check(nat.lib().jet_hip_fill_random(ctx.handle, nat.C64, n, 7, 2, buf.ptr, None))
+    g = from_device(buf, n, np.complex64)
+    assert abs(np.var(g.real) - 0.5) < 0.05 and abs(np.var(g.imag) - 0.5) < 0.05
+
+
+def test_reduce_compensated(ctx):
+    say("reduce")
+    rng = np.random.default_rng(1)
+    from hiputil import to_device, from_device
+    big = rng.standard_normal(3000) * 1e15
+    x = np.concatenate([big, rng.standard_normal(3000), -big]) * (1 + 0.5j)
+    rng.shuffle(x)
+    d = to_device(ctx, x)
+    out = nat.DeviceBuffer(16, ctx)
+    nat.check(nat.lib().jet_hip_reduce_compensated(ctx.handle, nat.C128, x.size, d.ptr, out.ptr,
+                                                   None))
+    got = from_device(out, 2, np.float64)
+    want = oracle.sum_compensated(x)
+    assert abs(got[0] - want.real) < 1e-6 and abs(got[1] - want.imag) < 1e-6
