@@ -1,0 +1,281 @@
+"""Benchmark of the MI355X contraction path (one JSON line on stdout).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1]
+
+Workloads:
+  c1  single large pairwise contraction: complex64 A, B of rank 28 (all
+      extents 2, 2^28 elements each), 14 shared indices at seeded random
+      positions in both index orders, re/im ~ N(0, 1/2); one
+      2^14 x 2^14 x 2^14 complex GEMM behind the index permutations.
+      A step is one ContractTensors call in split-BF16 3M (the headline
+      value); the plain FP32-MFMA path is timed beside it, and complex128 on
+      FP64 MFMA gives the FP64 reference the error is measured against.
+
+value = algorithmic TFLOP/s (8*M*N*K real flops per contraction) with the
+operands resident in HBM when the timed region starts.  The timed region is
+bracketed by HIP events on the library's stream and a full synchronization.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TFLOP/s & slices/s, 7x7 grid amp, split-BF16 MFMA vs plain FP32, err<=1e-5"
+PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# C1: one rank-28 x rank-28 contraction
+# ---------------------------------------------------------------------------
+
+def c1_problem(seed=2026, rank=28, shared=14):
+    rng = np.random.default_rng(seed)
+    la = [f"a{i}" for i in range(rank)]
+    common = [f"k{i}" for i in range(shared)]
+    pos_a = sorted(rng.choice(rank, size=shared, replace=False).tolist())
+    for j, p in enumerate(pos_a):
+        la[p] = common[j]
+    lb = [f"b{i}" for i in range(rank)]
+    perm = rng.permutation(shared).tolist()
+    pos_b = sorted(rng.choice(rank, size=shared, replace=False).tolist())
+    for j, p in enumerate(pos_b):
+        lb[p] = common[perm[j]]
+    return la, lb
+
+
+def _bit_offsets(labels, wanted):
+    """Linear-offset contribution of each label in `wanted` (all extents 2)."""
+    r = len(labels)
+    return np.array([1 << (r - 1 - labels.index(w)) for w in wanted], dtype=np.int64)
+
+
+def c1_sampled_reference(a_host, b_host, la, lb, rows, cols):
+    """Exact FP64 entries C[m, n] of the C1 contraction, from the unpermuted
+    operands (index arithmetic only; independent of the GPU plan)."""
+    left = [x for x in la if x not in lb]
+    right = [x for x in lb if x not in la]
+    common = [x for x in la if x in lb]
+    nk = len(common)
+    kbits = (np.arange(1 << nk)[:, None] >> np.arange(nk - 1, -1, -1)[None, :]) & 1
+    offA_k = kbits @ _bit_offsets(la, common)
+    offB_k = kbits @ _bit_offsets(lb, common)
+    offA_m = _bit_offsets(la, left)
+    offB_n = _bit_offsets(lb, right)
+    out = np.empty(len(rows), dtype=np.complex128)
+    for i, (m, n) in enumerate(zip(rows, cols)):
+        mb = (m >> np.arange(len(left) - 1, -1, -1)) & 1
+        nb = (n >> np.arange(len(right) - 1, -1, -1)) & 1
+        av = a_host[int(mb @ offA_m) + offA_k].astype(np.complex128)
+        bv = b_host[int(nb @ offB_n) + offB_k].astype(np.complex128)
+        out[i] = np.dot(av, bv)
+    return out
+
+
+def run_c1(args):
+    from jet_amd import _native as nat
+    from jet_amd.tensor import _relabel
+    import ctypes
+
+    ctx = nat.default_context()
+    L = nat.lib()
+    la, lb = c1_problem()
+    left = [x for x in la if x not in lb]
+    right = [x for x in lb if x not in la]
+    lc = left + right
+    ma, mb, mc = _relabel(la, lb, lc)
+    n = 1 << 28
+    ext = [2] * 28
+
+    A = nat.DeviceBuffer(n * 8, ctx)
+    B = nat.DeviceBuffer(n * 8, ctx)
+    C = nat.DeviceBuffer(n * 8, ctx)
+    nat.check(L.jet_hip_fill_random(ctx.handle, nat.C64, n, 1, 2, A.ptr, None))
+    nat.check(L.jet_hip_fill_random(ctx.handle, nat.C64, n, 2, 2, B.ptr, None))
+
+    def plan(dtype, prec):
+        p = ctypes.c_void_p()
+        nat.check(L.jet_hip_contraction_plan_create(ctx.handle, dtype, prec, nat.ROW_MAJOR, 28,
+                                                    nat.i32(ma), nat.i64(ext), 28, nat.i32(mb),
+                                                    nat.i64(ext), 28, nat.i32(mc),
+                                                    ctypes.byref(p)))
+        ws = ctypes.c_size_t()
+        nat.check(L.jet_hip_plan_workspace_size(p, ctypes.byref(ws)))
+        buf = nat.DeviceBuffer(max(ws.value, 256), ctx)
+        return p, buf, ws.value
+
+    flops = 8.0 * (1 << 42)
+
+    def timed(p, a, b, c, ws, wsz, steps, warmup, profile=False):
+        for _ in range(warmup):
+            nat.check(L.jet_hip_contract(p, a.ptr, b.ptr, c.ptr, ws.ptr, wsz, None))
+        ctx.synchronize()
+        if profile:
+            nat.Profiler.reset()
+            nat.Profiler.enable(True)
+        e0, e1 = nat.Event(ctx), nat.Event(ctx)
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(steps):
+            nat.check(L.jet_hip_contract(p, a.ptr, b.ptr, c.ptr, ws.ptr, wsz, None))
+        e1.record()
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        ms = e0.elapsed_ms(e1) / steps
+        prof = None
+        if profile:
+            prof = nat.Profiler.read()
+            nat.Profiler.enable(False)
+        return ms, wall, prof
+
+    log("c1: split-BF16 plan")
+    p16, ws16, wsz16 = plan(nat.C64, nat.PREC_BF16X3)
+    ms16, wall16, prof16 = timed(p16, A, B, C, ws16, wsz16, args.steps, args.warmup, True)
+    log(f"c1: bf16x3 {ms16:.2f} ms/step {flops / ms16 / 1e9:.1f} TFLOP/s")
+    c16 = np.empty(n, np.complex64)
+    C.download(c16)
+    del ws16
+
+    log("c1: FP32 plan")
+    p32, ws32, wsz32 = plan(nat.C64, nat.PREC_FP32)
+    ms32, _, prof32 = timed(p32, A, B, C, ws32, wsz32, max(1, args.steps // 2), 1, True)
+    log(f"c1: fp32 {ms32:.2f} ms/step {flops / ms32 / 1e9:.1f} TFLOP/s")
+    c32 = np.empty(n, np.complex64)
+    C.download(c32)
+    del ws32, C
+
+    log("c1: FP64 reference")
+    A64 = nat.DeviceBuffer(n * 16, ctx)
+    B64 = nat.DeviceBuffer(n * 16, ctx)
+    C64 = nat.DeviceBuffer(n * 16, ctx)
+    nat.check(L.jet_hip_convert(ctx.handle, nat.C64, nat.C128, n, A.ptr, A64.ptr, None))
+    nat.check(L.jet_hip_convert(ctx.handle, nat.C64, nat.C128, n, B.ptr, B64.ptr, None))
+    p64, ws64, wsz64 = plan(nat.C128, nat.PREC_FP64)
+    ms64, _, _ = timed(p64, A64, B64, C64, ws64, wsz64, 1, 0)
+    log(f"c1: fp64 {ms64:.2f} ms/step {flops / ms64 / 1e9:.1f} TFLOP/s")
+    c64ref = np.empty(n, np.complex128)
+    C64.download(c64ref)
+    del ws64, A64, B64, C64
+
+    # Pin the FP64 GPU reference to exact CPU dot products on sampled entries.
+    a_host = np.empty(n, np.complex64)
+    b_host = np.empty(n, np.complex64)
+    A.download(a_host)
+    B.download(b_host)
+    rng = np.random.default_rng(7)
+    rows = rng.integers(0, 1 << 14, 48)
+    cols = rng.integers(0, 1 << 14, 48)
+    exact = c1_sampled_reference(a_host, b_host, la, lb, rows, cols)
+    got64 = c64ref.reshape(1 << 14, 1 << 14)[rows, cols]
+    fp64_vs_cpu = float(np.max(np.abs(got64 - exact)) / np.sqrt(np.mean(np.abs(exact) ** 2)))
+    nref = np.linalg.norm(c64ref)
+    err16 = float(np.linalg.norm(c16.astype(np.complex128) - c64ref) / nref)
+    err32 = float(np.linalg.norm(c32.astype(np.complex128) - c64ref) / nref)
+    log(f"c1: err bf16x3 {err16:.3e} fp32 {err32:.3e} fp64-vs-cpu-sampled {fp64_vs_cpu:.2e}")
+    for p in (p16, p32, p64):
+        L.jet_hip_plan_destroy(p)
+
+    gemm = prof16.get("gemm_bf16x3_mfma", {})
+    gemm_ms = gemm.get("ms", 0.0) / max(1, gemm.get("launches", 1))
+    gemm_flops = gemm.get("flops", 0.0) / max(1, gemm.get("launches", 1))
+    achieved = gemm_flops / (gemm_ms * 1e-3) if gemm_ms else 0.0
+    result = {
+        "metric": METRIC,
+        "value": round(flops / (ms16 * 1e-3) / 1e12, 3),
+        "unit": "TFLOP/s",
+        "ms_per_step": round(ms16, 3),
+        "dtype": "c64 (split-bf16x3 MFMA, fp32 accumulate)",
+        "config": {"workload": "c1: rank-28 x rank-28 complex64 contraction, 14 shared "
+                               "indices at random positions, 2^14 x 2^14 x 2^14 complex GEMM",
+                   "m": 1 << 14, "n": 1 << 14, "k": 1 << 14},
+        "fp32_mfma": {"ms_per_step": round(ms32, 3),
+                      "tflops": round(flops / (ms32 * 1e-3) / 1e12, 3)},
+        "fp64_mfma": {"ms_per_step": round(ms64, 3),
+                      "tflops": round(flops / (ms64 * 1e-3) / 1e12, 3)},
+        "speedup_bf16x3_vs_fp32": round(ms32 / ms16, 3),
+        "rel_err_vs_fp64": {"bf16x3": err16, "fp32": err32,
+                            "fp64_vs_cpu_exact_sampled": fp64_vs_cpu},
+        "kernels_bf16x3": {k: {"launches": v["launches"],
+                               "ms_avg": round(v["ms"] / v["launches"], 4)}
+                           for k, v in prof16.items()},
+        "kernels_fp32": {k: {"launches": v["launches"],
+                             "ms_avg": round(v["ms"] / v["launches"], 4)}
+                         for k, v in prof32.items()},
+        "roofline": {"kernel": "gemm_bf16x3_mfma", "bound": "mfma",
+                     "achieved": round(achieved / 1e12, 3), "peak": PEAK["bf16"] / 1e12,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK["bf16"], 4),
+                     "hw_bf16_frac": round(achieved * 18 / 8 / PEAK["bf16"], 4),
+                     "traffic": None},
+    }
+    return result, (a_host, b_host, la, lb)
+
+
+def c1_cpu_baseline(host, threads):
+    """Reference CPU path (QFlex transposes + OpenBLAS cgemm) on a bounded
+    sample of C1: 4 of A's free indices fixed to 0, i.e. M = 2^10."""
+    import oracle
+    a_host, b_host, la, lb = host
+    left = [x for x in la if x not in lb]
+    fixed = left[:4]
+    shape = [2] * 28
+    arr = a_host.reshape(shape)
+    idx = tuple(0 if la[i] in fixed else slice(None) for i in range(28))
+    a_s = np.ascontiguousarray(arr[idx]).reshape(-1)
+    la_s = [x for x in la if x not in fixed]
+    oracle.load_blas(threads)
+    t0 = time.perf_counter()
+    _, c = oracle.baseline_contract(la_s, [2] * 24, a_s, lb, [2] * 28, b_host, threads)
+    dt = time.perf_counter() - t0
+    flops = 8.0 * (1 << 10) * (1 << 14) * (1 << 14)
+    return {"value": round(flops / dt / 1e12, 4), "unit": "TFLOP/s", "cores": threads,
+            "kind": "port",
+            "sample": "c1 with 4 free indices of A fixed (M=2^10, N=K=2^14): QFlex "
+                      "transposes + OpenBLAS cblas_cgemm, one ContractTensors call, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c1", choices=["c1"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the slice "
+              "loop shards across GPUs by contiguous slice ranges, which this build does "
+              "not run.", file=sys.stderr)
+        sys.exit(2)
+
+    result, host = run_c1(args)
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        result["cpu_baseline"] = c1_cpu_baseline(host, threads)
+    else:
+        result["cpu_baseline"] = None
+    result.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                   "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                   "data": "synthetic (seeded counter-based N(0,1/2))"})
+    order = ["metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
+             "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config"]
+    out = {k: result[k] for k in order if k in result}
+    out.update({k: v for k, v in result.items() if k not in out})
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

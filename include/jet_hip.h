@@ -177,6 +177,31 @@ int jet_hip_reduce_compensated(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
                                size_t n, const void *x, double *out,
                                void *stream);
 
+/* ---- dtype conversion (Tensor<complex<float>> <-> Tensor<complex<double>>
+ *      data, as used to build the FP64 oracle inputs on the device) ------- */
+int jet_hip_convert(jet_hip_ctx_t ctx, jet_hip_dtype_t src_dtype,
+                    jet_hip_dtype_t dst_dtype, size_t n, const void *in,
+                    void *out, void *stream);
+
+/* ---- events (timing on a stream; replaces the std::chrono timing around
+ *      Contract().wait() of the reference's benchmark drivers,
+ *      examples/paper_benchmarks/CPU/jet_cpu_m10/jet_sliced.cpp:85-91) ---- */
+int jet_hip_event_create(jet_hip_ctx_t ctx, void **event);
+int jet_hip_event_record(jet_hip_ctx_t ctx, void *event, void *stream);
+int jet_hip_event_elapsed_ms(void *start, void *stop, float *ms);
+int jet_hip_event_destroy(void *event);
+
+/* ---- per-kernel profiler (no counterpart in the reference, which has no
+ *      tracer).  When enabled, each kernel launch is bracketed by events on
+ *      its stream; jet_hip_profiler_read waits for them and returns, for each
+ *      kernel family k < *count, launches[k], total device ms[k] and the
+ *      algorithmic flops[k] / minimum HBM bytes[k] of those launches.      */
+int jet_hip_profiler_enable(int enable);
+int jet_hip_profiler_reset(void);
+int jet_hip_profiler_read(int capacity, int64_t *launches, double *ms,
+                          double *flops, double *bytes, int *count);
+const char *jet_hip_profiler_kernel_name(int kernel);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,7 +34,10 @@ EXPORTS = [
     "jet_hip_device_synchronize", "jet_hip_permute", "jet_hip_contraction_plan_create",
     "jet_hip_plan_workspace_size", "jet_hip_plan_info", "jet_hip_plan_destroy",
     "jet_hip_contract", "jet_hip_add", "jet_hip_slice", "jet_hip_fill_random",
-    "jet_hip_reduce_compensated",
+    "jet_hip_reduce_compensated", "jet_hip_convert", "jet_hip_event_create",
+    "jet_hip_event_record", "jet_hip_event_elapsed_ms", "jet_hip_event_destroy",
+    "jet_hip_profiler_enable", "jet_hip_profiler_reset", "jet_hip_profiler_read",
+    "jet_hip_profiler_kernel_name",
 ]
 
 
@@ -54,6 +57,8 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 def _declare(L):
     L.jet_hip_last_error.restype = ctypes.c_char_p
     L.jet_hip_last_error.argtypes = []
+    L.jet_hip_profiler_kernel_name.restype = ctypes.c_char_p
+    L.jet_hip_profiler_kernel_name.argtypes = [ctypes.c_int]
     sig = {
         "jet_hip_get_version": [ctypes.POINTER(ctypes.c_int)] * 3,
         "jet_hip_ctx_create": [ctypes.c_int, ctypes.POINTER(_vp)],
@@ -83,6 +88,16 @@ def _declare(L):
         "jet_hip_fill_random": [_vp, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int,
                                 _vp, _vp],
         "jet_hip_reduce_compensated": [_vp, ctypes.c_int, ctypes.c_size_t, _vp, _vp, _vp],
+        "jet_hip_convert": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, _vp, _vp, _vp],
+        "jet_hip_event_create": [_vp, ctypes.POINTER(_vp)],
+        "jet_hip_event_record": [_vp, _vp, _vp],
+        "jet_hip_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+        "jet_hip_event_destroy": [_vp],
+        "jet_hip_profiler_enable": [ctypes.c_int],
+        "jet_hip_profiler_reset": [],
+        "jet_hip_profiler_read": [ctypes.c_int, _i64p, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -186,3 +201,57 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class Event:
+    def __init__(self, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        h = _vp()
+        check(lib().jet_hip_event_create(self.ctx.handle, ctypes.byref(h)))
+        self.handle = h
+
+    def record(self, stream=None):
+        check(lib().jet_hip_event_record(self.ctx.handle, self.handle, stream))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = ctypes.c_float()
+        check(lib().jet_hip_event_elapsed_ms(self.handle, later.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if _lib is not None and self.handle:
+                lib().jet_hip_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Profiler:
+    """Per-kernel-family device time and algorithmic work (HIP events)."""
+
+    N = 32
+
+    @staticmethod
+    def enable(on: bool = True):
+        check(lib().jet_hip_profiler_enable(1 if on else 0))
+
+    @staticmethod
+    def reset():
+        check(lib().jet_hip_profiler_reset())
+
+    @staticmethod
+    def read() -> dict:
+        n = Profiler.N
+        launches = (ctypes.c_int64 * n)()
+        ms = (ctypes.c_double * n)()
+        flops = (ctypes.c_double * n)()
+        nbytes = (ctypes.c_double * n)()
+        count = ctypes.c_int()
+        check(lib().jet_hip_profiler_read(n, launches, ms, flops, nbytes, ctypes.byref(count)))
+        out = {}
+        for k in range(count.value):
+            if launches[k]:
+                name = lib().jet_hip_profiler_kernel_name(k).decode()
+                out[name] = dict(launches=int(launches[k]), ms=float(ms[k]),
+                                 flops=float(flops[k]), bytes=float(nbytes[k]))
+        return out

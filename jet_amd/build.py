@@ -25,6 +25,7 @@ SOURCES = [
     "permute.hip",
     "gemm.hip",
     "misc.hip",
+    "profiler.hip",
     "network.hip",
     "jet_hip_api.hip",
     "permute_plan.cpp",

@@ -21,6 +21,7 @@
 //  * VALU kernels for shapes too small or skinny for MFMA tiles.
 #include "common.hpp"
 #include "gemm.hpp"
+#include "profiler.hpp"
 
 #include <algorithm>
 
@@ -33,13 +34,19 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 template <class R> struct Vec;
+// Output row of accumulator item r held by lane group q (= lane / 16) in a
+// 16x16x4 MFMA block: v_mfma_f32_16x16x4_f32 returns D[4q + r][lane % 16],
+// v_mfma_f64_16x16x4_f64 returns D[q + 4r][lane % 16] (both measured on
+// gfx950).
 template <> struct Vec<float> {
     using c = float2;
     using acc = f32x4;
+    static __device__ __forceinline__ int row(int q, int r) { return 4 * q + r; }
 };
 template <> struct Vec<double> {
     using c = double2;
     using acc = f64x4;
+    static __device__ __forceinline__ int row(int q, int r) { return q + 4 * r; }
 };
 
 __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c)
@@ -56,7 +63,7 @@ __device__ __forceinline__ f64x4 mfma4(double a, double b, f64x4 c)
 // Block tile BM x BN complex outputs, BK complex k per LDS stage, 4 waves in
 // a WM x WN grid, double-buffered LDS with register prefetch.
 // Lane l of a 16x16x4 MFMA holds A[l%16][l/16] and B[l/16][l%16] and gets
-// D[4(l/16)+r][l%16]; here the four k-slots of four consecutive MFMAs cover
+// D[Vec<R>::row(l/16, r)][l%16]; here the four k-slots of four consecutive MFMAs cover
 // 8 complex k: lane slot q=l/16 of MFMA g is real k 4q+g, i.e. complex k
 // 2q+g/2, real part for even g and imaginary part for odd g.
 // ---------------------------------------------------------------------------
@@ -169,7 +176,7 @@ __global__ __launch_bounds__(256) void cgemm_embed_kernel(GemmArgs g)
             const int64_t col = n0 + wn * WTN + fn * 8 + (j >> 1);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t row = m0 + wm * WTM + fm * 16 + 4 * q + r;
+                const int64_t row = m0 + wm * WTM + fm * 16 + Vec<R>::row(q, r);
                 if (row < M && col < N) C[(row * g.ldc + col) * 2 + t] = acc[fm][fn][r];
             }
         }
@@ -445,10 +452,27 @@ __global__ __launch_bounds__(256) void cgemm_bf16x3_kernel(
     }
 }
 
+double gemm_flops(const GemmArgs &g)
+{
+    return 8.0 * double(g.batch) * double(g.M) * double(g.N) * double(g.K);
+}
+
+// Minimum HBM traffic: each operand read once, C written once (a broadcast
+// operand, batch stride 0, counts once).
+double gemm_bytes(const GemmArgs &g, int elem_bytes)
+{
+    const double a = double(g.M) * double(g.K) * (g.sA ? double(g.batch) : 1.0);
+    const double b = double(g.N) * double(g.K) * (g.sB ? double(g.batch) : 1.0);
+    const double c = double(g.M) * double(g.N) * double(g.batch);
+    return elem_bytes * (a + b + c);
+}
+
 template <class R>
 void launch_valu(int kernel, const GemmArgs &g, hipStream_t stream)
 {
     const int64_t total = g.batch * g.M * g.N;
+    prof::Scope ps(kernel == kGemmSmallRed ? prof::kGemmSmallRed : prof::kGemmSmall, stream,
+                   gemm_flops(g), gemm_bytes(g, int(sizeof(R) * 2)));
     if (kernel == kGemmSmallRed) {
         JH_REQUIRE(total <= (int64_t(1) << 31) - 1, "Too many outputs for the reduction kernel.");
         hipLaunchKernelGGL(cgemm_smallred_kernel<R>, dim3((unsigned)total), dim3(256), 0,
@@ -480,6 +504,7 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
         constexpr int BM = 128, BN = 128;
         JH_REQUIRE(ceil_div(g.N, BN) <= 65535, "N too large.");
         dim3 grid((unsigned)ceil_div(g.M, BM), (unsigned)ceil_div(g.N, BN), (unsigned)g.batch);
+        prof::Scope ps(prof::kGemmF32, stream, gemm_flops(g), gemm_bytes(g, 8));
         hipLaunchKernelGGL((cgemm_embed_kernel<float, BM, BN, 16, 2, 2>), grid, dim3(256), 0,
                            stream, g);
     }
@@ -488,6 +513,7 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
         constexpr int BM = 128, BN = 64;
         JH_REQUIRE(ceil_div(g.N, BN) <= 65535, "N too large.");
         dim3 grid((unsigned)ceil_div(g.M, BM), (unsigned)ceil_div(g.N, BN), (unsigned)g.batch);
+        prof::Scope ps(prof::kGemmF64, stream, gemm_flops(g), gemm_bytes(g, 16));
         hipLaunchKernelGGL((cgemm_embed_kernel<double, BM, BN, 8, 2, 2>), grid, dim3(256), 0,
                            stream, g);
     }
@@ -504,6 +530,11 @@ void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
     (void)batch;
     const int64_t total = dst.batch * dst.rows_p * (dst.k_p / 8);
     const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    // reads 8 bytes and writes 6 bf16 per (padded) element
+    prof::Scope ps(prof::kSplitBf16, stream, 0.0,
+                   8.0 * double(batch) * double(rows) * double(k) + 12.0 * double(dst.batch) *
+                                                                      double(dst.rows_p) *
+                                                                      double(dst.k_p));
     hipLaunchKernelGGL(split_bf16_kernel, dim3(grid), dim3(256), 0, stream,
                        static_cast<const float2 *>(src), rows, k, ld, s, dst);
     JH_HIP(hipGetLastError());
@@ -518,6 +549,9 @@ void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b,
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
     JH_REQUIRE(batch <= 65535 && ceil_div(N, BN) <= 65535, "Grid too large.");
     dim3 grid((unsigned)ceil_div(M, BM), (unsigned)ceil_div(N, BN), (unsigned)batch);
+    const double K = double(a.k_p);
+    prof::Scope ps(prof::kGemmBf16x3, stream, 8.0 * double(batch) * double(M) * double(N) * K,
+                   8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
     hipLaunchKernelGGL((cgemm_bf16x3_kernel<BM, BN, 2, 2>), grid, dim3(256), 0, stream, a, b,
                        static_cast<float2 *>(C), M, N, ldc, sC, a_bcast ? 1 : 0,
                        b_bcast ? 1 : 0);

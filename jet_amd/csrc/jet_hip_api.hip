@@ -7,6 +7,7 @@
 #include "gemm.hpp"
 #include "misc.hpp"
 #include "permute_plan.hpp"
+#include "profiler.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -471,5 +472,74 @@ int jet_hip_reduce_compensated(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, size_t 
         launch_reduce(elem_bytes_of(dtype), x, (int64_t)n, out, pick_stream(ctx, stream));
     });
 }
+
+int jet_hip_convert(jet_hip_ctx_t ctx, jet_hip_dtype_t src_dtype, jet_hip_dtype_t dst_dtype,
+                    size_t n, const void *in, void *out, void *stream)
+{
+    return guarded([&] {
+        set_device(ctx);
+        launch_convert(elem_bytes_of(src_dtype), elem_bytes_of(dst_dtype), in, out, (int64_t)n,
+                       pick_stream(ctx, stream));
+    });
+}
+
+int jet_hip_event_create(jet_hip_ctx_t ctx, void **event)
+{
+    return guarded([&] {
+        set_device(ctx);
+        JH_REQUIRE(event != nullptr, "Null output pointer.");
+        hipEvent_t e;
+        JH_HIP(hipEventCreate(&e));
+        *event = e;
+    });
+}
+
+int jet_hip_event_record(jet_hip_ctx_t ctx, void *event, void *stream)
+{
+    return guarded([&] {
+        set_device(ctx);
+        JH_HIP(hipEventRecord(static_cast<hipEvent_t>(event), pick_stream(ctx, stream)));
+    });
+}
+
+int jet_hip_event_elapsed_ms(void *start, void *stop, float *ms)
+{
+    return guarded([&] {
+        JH_REQUIRE(start && stop && ms, "Null argument.");
+        JH_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+        JH_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
+                                   static_cast<hipEvent_t>(stop)));
+    });
+}
+
+int jet_hip_event_destroy(void *event)
+{
+    return guarded([&] {
+        if (event) JH_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    });
+}
+
+int jet_hip_profiler_enable(int enable)
+{
+    prof::set_enabled(enable != 0);
+    return JET_HIP_SUCCESS;
+}
+
+int jet_hip_profiler_reset(void)
+{
+    return guarded([&] { prof::reset(); });
+}
+
+int jet_hip_profiler_read(int capacity, int64_t *launches, double *ms, double *flops,
+                          double *bytes, int *count)
+{
+    return guarded([&] {
+        JH_REQUIRE(capacity >= prof::kNumKernels, "Profiler buffers are too small.");
+        prof::read(launches, ms, flops, bytes);
+        if (count) *count = prof::kNumKernels;
+    });
+}
+
+const char *jet_hip_profiler_kernel_name(int kernel) { return prof::kernel_name(kernel); }
 
 } // extern "C"

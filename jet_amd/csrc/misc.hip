@@ -2,6 +2,7 @@
 // reduction (gfx950).
 #include "common.hpp"
 #include "misc.hpp"
+#include "profiler.hpp"
 
 #include <algorithm>
 
@@ -87,6 +88,18 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const C2 *__restrict__ x, 
     }
 }
 
+template <class S, class D>
+__global__ __launch_bounds__(256) void convert_kernel(const S *__restrict__ in,
+                                                      D *__restrict__ out, int64_t n)
+{
+    using R = decltype(D{}.x);
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const S v = in[i];
+        out[i] = D{R(v.x), R(v.y)};
+    }
+}
+
 int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
 
 } // namespace
@@ -95,6 +108,7 @@ void launch_add(int elem_bytes, const void *a, const void *b, void *c, int64_t n
                 hipStream_t stream)
 {
     if (n == 0) return;
+    prof::Scope ps(prof::kAdd, stream, 2.0 * double(n), 3.0 * elem_bytes * double(n));
     if (elem_bytes == 8)
         hipLaunchKernelGGL(add_kernel<float2>, dim3(grid_for(n)), dim3(256), 0, stream,
                            static_cast<const float2 *>(a), static_cast<const float2 *>(b),
@@ -111,6 +125,7 @@ void launch_fill(int elem_bytes, void *out, int64_t n, uint64_t seed, int dist,
 {
     if (n == 0) return;
     JH_REQUIRE(dist >= 0 && dist <= 2, "Unknown distribution.");
+    prof::Scope ps(prof::kFill, stream, 0.0, double(elem_bytes) * double(n));
     if (elem_bytes == 8)
         hipLaunchKernelGGL(fill_kernel<float2>, dim3(grid_for(n)), dim3(256), 0, stream,
                            static_cast<float2 *>(out), n, seed, dist);
@@ -120,9 +135,26 @@ void launch_fill(int elem_bytes, void *out, int64_t n, uint64_t seed, int dist,
     JH_HIP(hipGetLastError());
 }
 
+void launch_convert(int src_bytes, int dst_bytes, const void *in, void *out, int64_t n,
+                    hipStream_t stream)
+{
+    if (n == 0) return;
+    prof::Scope ps(prof::kConvert, stream, 0.0, double(src_bytes + dst_bytes) * double(n));
+    if (src_bytes == 8 && dst_bytes == 16)
+        hipLaunchKernelGGL((convert_kernel<float2, double2>), dim3(grid_for(n)), dim3(256), 0,
+                           stream, static_cast<const float2 *>(in), static_cast<double2 *>(out), n);
+    else if (src_bytes == 16 && dst_bytes == 8)
+        hipLaunchKernelGGL((convert_kernel<double2, float2>), dim3(grid_for(n)), dim3(256), 0,
+                           stream, static_cast<const double2 *>(in), static_cast<float2 *>(out), n);
+    else
+        JH_FAIL(JET_HIP_ERROR_INVALID_VALUE, "Conversion needs two different data types.");
+    JH_HIP(hipGetLastError());
+}
+
 void launch_reduce(int elem_bytes, const void *x, int64_t n, double *out,
                    hipStream_t stream)
 {
+    prof::Scope ps(prof::kReduce, stream, 8.0 * double(n), double(elem_bytes) * double(n));
     if (elem_bytes == 8)
         hipLaunchKernelGGL(reduce_kernel<float2>, dim3(1), dim3(1024), 0, stream,
                            static_cast<const float2 *>(x), n, out);

@@ -54,6 +54,9 @@ void launch_fill(int elem_bytes, void *out, int64_t n, uint64_t seed, int dist,
                  hipStream_t stream);
 void launch_reduce(int elem_bytes, const void *x, int64_t n, double *out,
                    hipStream_t stream);
+// complex64 <-> complex128 (rounding to nearest when narrowing).
+void launch_convert(int src_bytes, int dst_bytes, const void *in, void *out,
+                    int64_t n, hipStream_t stream);
 #endif
 
 } // namespace jet_hip

@@ -1,6 +1,7 @@
 // permute.hip -- LDS-tiled bit-permutation and gather kernels (gfx950).
 #include "common.hpp"
 #include "permute_plan.hpp"
+#include "profiler.hpp"
 
 #include <algorithm>
 #include <numeric>
@@ -66,12 +67,15 @@ template <class T>
 static void launch_typed(const PermutePlan &plan, const void *in, void *out,
                          hipStream_t stream)
 {
+    const double bytes = 2.0 * sizeof(T) * double(plan.numel);
     if (plan.kind == PermutePlan::kCopy) {
+        prof::Scope ps(prof::kCopy, stream, 0.0, bytes);
         JH_HIP(hipMemcpyAsync(out, in, sizeof(T) * plan.numel,
                               hipMemcpyDeviceToDevice, stream));
         return;
     }
     if (plan.kind == PermutePlan::kBits) {
+        prof::Scope ps(prof::kPermuteBits, stream, 0.0, bytes);
         const int64_t ntiles = int64_t(1) << plan.bits.nr;
         const int grid = (int)std::min<int64_t>(ntiles, 8192);
         const size_t lds = sizeof(T) << plan.bits.tb;
@@ -82,6 +86,7 @@ static void launch_typed(const PermutePlan &plan, const void *in, void *out,
         return;
     }
     const int grid = (int)std::min<int64_t>((plan.numel + 255) / 256, 16384);
+    prof::Scope ps(prof::kPermuteGeneric, stream, 0.0, bytes);
     hipLaunchKernelGGL(permute_generic_kernel<T>, dim3(grid), dim3(256), 0, stream,
                        static_cast<const T *>(in), static_cast<T *>(out), plan.gen,
                        plan.numel);

@@ -84,6 +84,12 @@ def lib():
                                   ctypes.c_int]
         L.jo_sum_compensated.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
         L.jo_sum_compensated.restype = None
+        L.jb_load_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.jb_set_blas_threads.argtypes = [ctypes.c_int]
+        L.jb_set_blas_threads.restype = None
+        L.jb_contract.argtypes = [ctypes.c_size_t, ctypes.c_int, i32p, i64p, ctypes.c_void_p,
+                                  ctypes.c_int, i32p, i64p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_int]
         _lib = L
     return _lib
 
@@ -321,3 +327,62 @@ def contract_path(tensors: List[Tuple[Sequence[str], Sequence[int], np.ndarray]]
         lb, sb, b = nodes[j]
         nodes.append(contract(la, sa, a, lb, sb, b, nthreads))
     return nodes[-1]
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the reference CPU path (QFlex transposes + OpenBLAS CBLAS),
+# restated in cpu_baseline.c.  Used only by bench.py's cpu_baseline leg.
+# ---------------------------------------------------------------------------
+
+_blas_path = None
+
+
+def find_openblas() -> Optional[Tuple[str, str]]:
+    """(path, symbol prefix) of an OpenBLAS with CBLAS symbols, or None."""
+    import glob
+    cands = []
+    try:
+        import scipy
+        cands += [(p, "scipy_") for p in glob.glob(os.path.join(
+            os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs", "libscipy_openblas*.so"))]
+    except ImportError:
+        pass
+    cands += [(p, "") for p in glob.glob("/usr/lib/x86_64-linux-gnu/libopenblas*.so*")]
+    return cands[0] if cands else None
+
+
+def load_blas(threads: int) -> str:
+    global _blas_path
+    if _blas_path is None:
+        found = find_openblas()
+        if found is None:
+            raise OracleError("no OpenBLAS with CBLAS symbols found")
+        rc = lib().jb_load_blas(found[0].encode(), found[1].encode())
+        if rc:
+            raise OracleError(f"could not load CBLAS from {found[0]} ({rc})")
+        _blas_path = found[0]
+    lib().jb_set_blas_threads(int(threads))
+    return _blas_path
+
+
+def baseline_contract(a_labels, a_shape, a, b_labels, b_shape, b, nthreads: int = 1):
+    """Reference CPU ContractTensors restated (output in the input dtype)."""
+    tab = LabelTable()
+    la = [tab.id(s) for s in a_labels]
+    lb = [tab.id(s) for s in b_labels]
+    left = vector_subtraction(a_labels, b_labels)
+    right = vector_subtraction(b_labels, a_labels)
+    size = 1
+    for s in left:
+        size *= a_shape[list(a_labels).index(s)]
+    for s in right:
+        size *= b_shape[list(b_labels).index(s)]
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    C = np.empty(size, dtype=a.dtype)
+    err = lib().jb_contract(a.itemsize, len(la), _i32(la), _i64(a_shape), a.ctypes.data,
+                            len(lb), _i32(lb), _i64(b_shape), b.ctypes.data, C.ctypes.data,
+                            int(nthreads))
+    if err:
+        raise OracleError(_ERRORS.get(err, str(err)))
+    return left + right, C
