@@ -69,6 +69,7 @@ typedef enum {
 
 typedef struct jet_hip_ctx_s *jet_hip_ctx_t;
 typedef struct jet_hip_plan_s *jet_hip_plan_t;
+typedef struct jet_hip_network_s *jet_hip_network_t;
 
 /* ---- library / errors -------------------------------------------------- */
 
@@ -176,6 +177,59 @@ int jet_hip_fill_random(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, size_t n,
 int jet_hip_reduce_compensated(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
                                size_t n, const void *x, double *out,
                                void *stream);
+
+/* ---- contraction sequences (replaces the per-task ContractTensors calls
+ *      that TaskBasedContractor schedules, include/jet/TaskBasedContractor.hpp:
+ *      162-248, 382-394, and TensorNetwork::Contract(path),
+ *      include/jet/TensorNetwork.hpp:301-328, for a whole path at once)
+ *
+ * Leaves: n_leaves row-major tensors; leaf i has leaf_ranks[i] modes, and the
+ * modes / extents of all leaves are concatenated in leaf_modes / leaf_extents.
+ * Path: n_steps pairs (path[2s], path[2s+1]) of SSA ids -- leaves are
+ * 0..n_leaves-1 and step s produces node n_leaves+s (PathInfo's convention,
+ * include/jet/PathInfo.hpp:262-297).
+ * out_modes: the modes of the root, in the order it is written.  A mode in
+ * out_modes is never summed by a step, even where both operands hold it: it
+ * then becomes a batch mode.  Listing the sliced indices of a sliced network
+ * there runs every slice in one sequence and leaves the per-slice results
+ * in the root (the batched slice loop).  Modes held by more than two leaves
+ * are summed once no remaining tensor holds them.
+ * The network owns its intermediate arena and workspace (device memory,
+ * allocated at creation).  Executions on one network must not overlap.    */
+int jet_hip_network_create(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
+                           jet_hip_precision_t precision, int n_leaves,
+                           const int32_t *leaf_ranks, const int32_t *leaf_modes,
+                           const int64_t *leaf_extents, int n_steps,
+                           const int32_t *path, int n_out,
+                           const int32_t *out_modes, jet_hip_network_t *net);
+/* The plan of jet_hip_network_create without a device or any allocation:
+ * algorithmic flops and the arena / workspace bytes it would allocate. */
+int jet_hip_network_estimate(jet_hip_dtype_t dtype,
+                             jet_hip_precision_t precision, int n_leaves,
+                             const int32_t *leaf_ranks,
+                             const int32_t *leaf_modes,
+                             const int64_t *leaf_extents, int n_steps,
+                             const int32_t *path, int n_out,
+                             const int32_t *out_modes, double *flops,
+                             size_t *arena_bytes, size_t *workspace_bytes);
+/* Algorithmic real flops (sum of 8*M*N*K*batch), arena and workspace bytes,
+ * root size in elements, number of steps. */
+int jet_hip_network_info(jet_hip_network_t net, double *flops,
+                         size_t *arena_bytes, size_t *workspace_bytes,
+                         int64_t *out_size, int *n_steps);
+/* GEMM shape, kernel and permutations (bit 0: A, 1: B, 2: C) of one step. */
+int jet_hip_network_step_info(jet_hip_network_t net, int step, int64_t *m,
+                              int64_t *n, int64_t *k, int64_t *batch,
+                              int *gemm_kernel, int *permutes);
+int jet_hip_network_execute(jet_hip_network_t net, const void *const *leaves,
+                            void *out, void *stream);
+/* Capture one execution with these leaf and output pointers as a hipGraph
+ * (replaces the cudaFlow capture of examples/heterogeneous_contraction.cu:
+ * 91-142); jet_hip_network_launch replays it. */
+int jet_hip_network_capture(jet_hip_network_t net, const void *const *leaves,
+                            void *out, void *stream);
+int jet_hip_network_launch(jet_hip_network_t net, void *stream);
+int jet_hip_network_destroy(jet_hip_network_t net);
 
 /* ---- dtype conversion (Tensor<complex<float>> <-> Tensor<complex<double>>
  *      data, as used to build the FP64 oracle inputs on the device) ------- */

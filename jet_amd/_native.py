@@ -37,7 +37,9 @@ EXPORTS = [
     "jet_hip_reduce_compensated", "jet_hip_convert", "jet_hip_event_create",
     "jet_hip_event_record", "jet_hip_event_elapsed_ms", "jet_hip_event_destroy",
     "jet_hip_profiler_enable", "jet_hip_profiler_reset", "jet_hip_profiler_read",
-    "jet_hip_profiler_kernel_name",
+    "jet_hip_profiler_kernel_name", "jet_hip_network_create", "jet_hip_network_estimate", "jet_hip_network_info",
+    "jet_hip_network_step_info", "jet_hip_network_execute", "jet_hip_network_capture",
+    "jet_hip_network_launch", "jet_hip_network_destroy",
 ]
 
 
@@ -93,6 +95,23 @@ def _declare(L):
         "jet_hip_event_record": [_vp, _vp, _vp],
         "jet_hip_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
         "jet_hip_event_destroy": [_vp],
+        "jet_hip_network_create": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
+                                   _i64p, ctypes.c_int, _i32p, ctypes.c_int, _i32p,
+                                   ctypes.POINTER(_vp)],
+        "jet_hip_network_estimate": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
+                                     _i64p, ctypes.c_int, _i32p, ctypes.c_int, _i32p,
+                                     ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_size_t),
+                                     ctypes.POINTER(ctypes.c_size_t)],
+        "jet_hip_network_info": [_vp, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                                 _i64p, ctypes.POINTER(ctypes.c_int)],
+        "jet_hip_network_step_info": [_vp, ctypes.c_int, _i64p, _i64p, _i64p, _i64p,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+        "jet_hip_network_execute": [_vp, ctypes.POINTER(_vp), _vp, _vp],
+        "jet_hip_network_capture": [_vp, ctypes.POINTER(_vp), _vp, _vp],
+        "jet_hip_network_launch": [_vp, _vp],
+        "jet_hip_network_destroy": [_vp],
         "jet_hip_profiler_enable": [ctypes.c_int],
         "jet_hip_profiler_reset": [],
         "jet_hip_profiler_read": [ctypes.c_int, _i64p, ctypes.POINTER(ctypes.c_double),

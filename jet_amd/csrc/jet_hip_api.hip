@@ -6,6 +6,7 @@
 #include "contract_plan.hpp"
 #include "gemm.hpp"
 #include "misc.hpp"
+#include "network_plan.hpp"
 #include "permute_plan.hpp"
 #include "profiler.hpp"
 
@@ -26,6 +27,15 @@ struct jet_hip_ctx_s {
 struct jet_hip_plan_s {
     jet_hip_ctx_s *ctx = nullptr;
     ContractionDesc d;
+};
+
+struct jet_hip_network_s {
+    jet_hip_ctx_s *ctx = nullptr;
+    NetworkPlan plan;
+    char *arena = nullptr;
+    char *workspace = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 };
 
 namespace {
@@ -158,6 +168,34 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         launch_gemm(d.gemm_kernel, eb, g, s);
     }
     if (d.need_c) launch_permute(d.pc, eb, Ct, c, s);
+}
+
+void run_network(jet_hip_network_s *net, const void *const *leaves, void *out, hipStream_t s)
+{
+    const NetworkPlan &np = net->plan;
+    if (np.steps.empty()) {
+        launch_permute(np.root_copy, np.elem_bytes, leaves[0], out, s);
+        return;
+    }
+    std::vector<const void *> node(np.n_leaves + np.steps.size(), nullptr);
+    for (int i = 0; i < np.n_leaves; i++) {
+        JH_REQUIRE(leaves[i] != nullptr, "Null leaf tensor.");
+        node[i] = leaves[i];
+    }
+    for (size_t k = 0; k < np.steps.size(); k++) {
+        const NetworkStep &st = np.steps[k];
+        void *c = st.is_root ? out : static_cast<void *>(net->arena + st.offset);
+        run_contraction(st.desc, node[st.a], node[st.b], c, net->workspace, s);
+        node[np.n_leaves + k] = c;
+    }
+}
+
+void free_graph(jet_hip_network_s *net)
+{
+    if (net->graph_exec) (void)hipGraphExecDestroy(net->graph_exec);
+    if (net->graph) (void)hipGraphDestroy(net->graph);
+    net->graph_exec = nullptr;
+    net->graph = nullptr;
 }
 
 } // namespace
@@ -541,5 +579,147 @@ int jet_hip_profiler_read(int capacity, int64_t *launches, double *ms, double *f
 }
 
 const char *jet_hip_profiler_kernel_name(int kernel) { return prof::kernel_name(kernel); }
+
+int jet_hip_network_create(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, jet_hip_precision_t precision,
+                           int n_leaves, const int32_t *leaf_ranks, const int32_t *leaf_modes,
+                           const int64_t *leaf_extents, int n_steps, const int32_t *path,
+                           int n_out, const int32_t *out_modes, jet_hip_network_t *net)
+{
+    return guarded([&] {
+        JH_REQUIRE(ctx && net, "Null argument.");
+        set_device(ctx);
+        auto *n = new jet_hip_network_s;
+        try {
+            n->ctx = ctx;
+            n->plan = plan_network(elem_bytes_of(dtype), mode_of(dtype, precision), n_leaves,
+                                   leaf_ranks, leaf_modes, leaf_extents, n_steps, path, n_out,
+                                   out_modes);
+            if (n->plan.arena_bytes) {
+                void *p = nullptr;
+                JH_HIP(hipMalloc(&p, n->plan.arena_bytes));
+                n->arena = static_cast<char *>(p);
+            }
+            if (n->plan.workspace_bytes) {
+                void *p = nullptr;
+                JH_HIP(hipMalloc(&p, n->plan.workspace_bytes));
+                n->workspace = static_cast<char *>(p);
+            }
+        }
+        catch (...) {
+            if (n->arena) (void)hipFree(n->arena);
+            if (n->workspace) (void)hipFree(n->workspace);
+            delete n;
+            throw;
+        }
+        *net = n;
+    });
+}
+
+int jet_hip_network_estimate(jet_hip_dtype_t dtype, jet_hip_precision_t precision, int n_leaves,
+                             const int32_t *leaf_ranks, const int32_t *leaf_modes,
+                             const int64_t *leaf_extents, int n_steps, const int32_t *path,
+                             int n_out, const int32_t *out_modes, double *flops,
+                             size_t *arena_bytes, size_t *workspace_bytes)
+{
+    return guarded([&] {
+        NetworkPlan np = plan_network(elem_bytes_of(dtype), mode_of(dtype, precision), n_leaves,
+                                      leaf_ranks, leaf_modes, leaf_extents, n_steps, path, n_out,
+                                      out_modes);
+        if (flops) *flops = np.flops;
+        if (arena_bytes) *arena_bytes = np.arena_bytes;
+        if (workspace_bytes) *workspace_bytes = np.workspace_bytes;
+    });
+}
+
+int jet_hip_network_info(jet_hip_network_t net, double *flops, size_t *arena_bytes,
+                         size_t *workspace_bytes, int64_t *out_size, int *n_steps)
+{
+    return guarded([&] {
+        JH_REQUIRE(net, "Null network.");
+        if (flops) *flops = net->plan.flops;
+        if (arena_bytes) *arena_bytes = net->plan.arena_bytes;
+        if (workspace_bytes) *workspace_bytes = net->plan.workspace_bytes;
+        if (out_size) {
+            int64_t n = 1;
+            for (auto e : net->plan.out_extents) n *= e;
+            *out_size = n;
+        }
+        if (n_steps) *n_steps = (int)net->plan.steps.size();
+    });
+}
+
+int jet_hip_network_step_info(jet_hip_network_t net, int step, int64_t *m, int64_t *n, int64_t *k,
+                              int64_t *batch, int *gemm_kernel, int *permutes)
+{
+    return guarded([&] {
+        JH_REQUIRE(net && step >= 0 && step < (int)net->plan.steps.size(), "Invalid step.");
+        const ContractionDesc &d = net->plan.steps[step].desc;
+        if (m) *m = d.M;
+        if (n) *n = d.N;
+        if (k) *k = d.K;
+        if (batch) *batch = d.batch;
+        if (gemm_kernel) *gemm_kernel = d.gemm_kernel;
+        if (permutes) *permutes = (d.need_a ? 1 : 0) | (d.need_b ? 2 : 0) | (d.need_c ? 4 : 0);
+    });
+}
+
+int jet_hip_network_execute(jet_hip_network_t net, const void *const *leaves, void *out,
+                            void *stream)
+{
+    return guarded([&] {
+        JH_REQUIRE(net && leaves && out, "Null argument.");
+        set_device(net->ctx);
+        run_network(net, leaves, out, pick_stream(net->ctx, stream));
+    });
+}
+
+int jet_hip_network_capture(jet_hip_network_t net, const void *const *leaves, void *out,
+                            void *stream)
+{
+    return guarded([&] {
+        JH_REQUIRE(net && leaves && out, "Null argument.");
+        set_device(net->ctx);
+        hipStream_t s = pick_stream(net->ctx, stream);
+        free_graph(net);
+        const bool prof_on = prof::enabled();
+        prof::set_enabled(false);
+        JH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            run_network(net, leaves, out, s);
+        }
+        catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            prof::set_enabled(prof_on);
+            throw;
+        }
+        prof::set_enabled(prof_on);
+        JH_HIP(hipStreamEndCapture(s, &net->graph));
+        JH_HIP(hipGraphInstantiate(&net->graph_exec, net->graph, nullptr, nullptr, 0));
+    });
+}
+
+int jet_hip_network_launch(jet_hip_network_t net, void *stream)
+{
+    return guarded([&] {
+        JH_REQUIRE(net && net->graph_exec, "The network has no captured graph.");
+        set_device(net->ctx);
+        JH_HIP(hipGraphLaunch(net->graph_exec, pick_stream(net->ctx, stream)));
+    });
+}
+
+int jet_hip_network_destroy(jet_hip_network_t net)
+{
+    return guarded([&] {
+        if (!net) return;
+        set_device(net->ctx);
+        JH_HIP(hipDeviceSynchronize());
+        free_graph(net);
+        if (net->arena) JH_HIP(hipFree(net->arena));
+        if (net->workspace) JH_HIP(hipFree(net->workspace));
+        delete net;
+    });
+}
 
 } // extern "C"

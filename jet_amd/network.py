@@ -1,0 +1,269 @@
+"""Device contraction sequences and the batched slice loop.
+
+``DeviceNetwork`` wraps ``jet_hip_network_*``: a whole contraction path over
+device tensors planned once (one ContractionDesc per step, one arena, one
+workspace) and executed in order on one stream or replayed as a hipGraph.
+
+``SlicedContraction`` is the MI355X form of the reference's slice loop
+(examples/paper_benchmarks/CPU/jet_cpu_m10/jet_sliced.cpp:53-93 driving
+TensorNetwork::SliceIndices, TensorNetwork.hpp:210-284, and
+TaskBasedContractor::AddContractionTasks / AddReductionTask,
+TaskBasedContractor.hpp:162-280).  The sliced indices split in two groups:
+
+  * batch indices stay in the network as modes that no step sums (they become
+    batch modes of the GEMMs where both operands hold them), so one execution
+    computes every combination of their values, and subtrees that hold none
+    of them are computed once -- the sharing that TaskBasedContractor gets
+    from identical task names;
+  * loop indices are fixed per execution by slicing the leaves that hold them
+    (Tensor::SliceIndex, Tensor.hpp:493-526), one execution per value
+    combination.
+
+The per-slice results land in one device vector in the reference's slice
+order (UnravelIndex over the sliced indices, first index most significant,
+Utilities.hpp:463-475; loop indices first, then batch indices) and are
+summed by the FP64 compensated reduction.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as nat
+
+_PREC = {None: nat.PREC_DEFAULT, "default": nat.PREC_DEFAULT, "fp32": nat.PREC_FP32,
+         "bf16x3": nat.PREC_BF16X3, "fp64": nat.PREC_FP64}
+
+
+class LabelIds:
+    def __init__(self):
+        self.ids: Dict[str, int] = {}
+
+    def __call__(self, s: str) -> int:
+        if s not in self.ids:
+            self.ids[s] = len(self.ids)
+        return self.ids[s]
+
+
+def _flatten(indices, shapes, lab):
+    ranks = [len(ix) for ix in indices]
+    modes = [lab(l) for ix in indices for l in ix]
+    exts = [int(s) for sh in shapes for s in sh]
+    return ranks, modes, exts
+
+
+def estimate(indices: Sequence[Sequence[str]], shapes, path, out_labels, dtype=np.complex64,
+             precision=None) -> Tuple[float, int, int]:
+    """(flops, arena bytes, workspace bytes) of a network plan; no device."""
+    lab = LabelIds()
+    ranks, modes, exts = _flatten(indices, shapes, lab)
+    out = [lab(l) for l in out_labels]
+    flat_path = [int(x) for p in path for x in p]
+    fl = ctypes.c_double()
+    ar, ws = ctypes.c_size_t(), ctypes.c_size_t()
+    code = nat.C64 if np.dtype(dtype) == np.complex64 else nat.C128
+    nat.check(nat.lib().jet_hip_network_estimate(
+        code, _PREC[precision], len(indices), nat.i32(ranks), nat.i32(modes), nat.i64(exts),
+        len(path), nat.i32(flat_path), len(out), nat.i32(out), ctypes.byref(fl),
+        ctypes.byref(ar), ctypes.byref(ws)))
+    return fl.value, ar.value, ws.value
+
+
+class DeviceNetwork:
+    """A planned contraction path over device tensors."""
+
+    def __init__(self, indices: Sequence[Sequence[str]], shapes, path, out_labels,
+                 dtype=np.complex64, precision: Optional[str] = None, ctx=None):
+        self.ctx = ctx or nat.default_context()
+        self.dtype = np.dtype(dtype)
+        self.code = nat.C64 if self.dtype == np.complex64 else nat.C128
+        lab = LabelIds()
+        ranks, modes, exts = _flatten(indices, shapes, lab)
+        self.out_labels = list(out_labels)
+        out = [lab(l) for l in self.out_labels]
+        flat_path = [int(x) for p in path for x in p]
+        self.n_leaves = len(indices)
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().jet_hip_network_create(
+            self.ctx.handle, self.code, _PREC[precision], len(indices), nat.i32(ranks),
+            nat.i32(modes), nat.i64(exts), len(path), nat.i32(flat_path), len(out), nat.i32(out),
+            ctypes.byref(h)))
+        self.handle = h
+        fl = ctypes.c_double()
+        ar, ws = ctypes.c_size_t(), ctypes.c_size_t()
+        osz = ctypes.c_int64()
+        ns = ctypes.c_int()
+        nat.check(nat.lib().jet_hip_network_info(h, ctypes.byref(fl), ctypes.byref(ar),
+                                                 ctypes.byref(ws), ctypes.byref(osz),
+                                                 ctypes.byref(ns)))
+        self.flops, self.arena_bytes, self.workspace_bytes = fl.value, ar.value, ws.value
+        self.out_size, self.n_steps = osz.value, ns.value
+        self._graph_key = None
+
+    def step_info(self, s: int) -> dict:
+        m, n, k, b = (ctypes.c_int64() for _ in range(4))
+        kern, perm = ctypes.c_int(), ctypes.c_int()
+        nat.check(nat.lib().jet_hip_network_step_info(self.handle, s, ctypes.byref(m),
+                                                      ctypes.byref(n), ctypes.byref(k),
+                                                      ctypes.byref(b), ctypes.byref(kern),
+                                                      ctypes.byref(perm)))
+        return dict(M=m.value, N=n.value, K=k.value, batch=b.value,
+                    kernel=nat.GEMM_KERNEL_NAMES[kern.value], permutes=perm.value)
+
+    def _ptrs(self, leaves):
+        assert len(leaves) == self.n_leaves
+        return (ctypes.c_void_p * self.n_leaves)(*[int(x) for x in leaves])
+
+    def execute(self, leaf_ptrs: Sequence[int], out_ptr: int, stream=None) -> None:
+        nat.check(nat.lib().jet_hip_network_execute(self.handle, self._ptrs(leaf_ptrs), out_ptr,
+                                                    stream))
+
+    def launch(self, leaf_ptrs: Sequence[int], out_ptr: int, stream=None) -> None:
+        """Replay as a hipGraph (captured on first use for these pointers)."""
+        key = (tuple(int(x) for x in leaf_ptrs), int(out_ptr))
+        if key != self._graph_key:
+            nat.check(nat.lib().jet_hip_network_capture(self.handle, self._ptrs(leaf_ptrs),
+                                                        out_ptr, stream))
+            self._graph_key = key
+        nat.check(nat.lib().jet_hip_network_launch(self.handle, stream))
+
+    def close(self) -> None:
+        if self.handle:
+            nat.check(nat.lib().jet_hip_network_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _slice_leaf(labels, shape, data, fixed: Dict[str, int]):
+    arr = np.asarray(data).reshape(shape) if shape else np.asarray(data).reshape(())
+    keep_l, keep_s, idx = [], [], []
+    for l, s in zip(labels, shape):
+        if l in fixed:
+            idx.append(fixed[l])
+        else:
+            idx.append(slice(None))
+            keep_l.append(l)
+            keep_s.append(s)
+    out = arr[tuple(idx)] if idx else arr
+    return keep_l, keep_s, np.ascontiguousarray(out).reshape(-1)
+
+
+class SlicedContraction:
+    """All slices of a sliced network on one GPU (see the module docstring)."""
+
+    def __init__(self, indices, shapes, data, path, sliced: Sequence[str],
+                 batch: Optional[Sequence[str]] = None, dtype=np.complex64,
+                 precision: Optional[str] = None, ctx=None, use_graph: bool = True):
+        self.ctx = ctx or nat.default_context()
+        self.dtype = np.dtype(dtype)
+        self.code = nat.C64 if self.dtype == np.complex64 else nat.C128
+        sliced = list(sliced)
+        batch = list(sliced if batch is None else batch)
+        assert set(batch) <= set(sliced)
+        self.loop = [l for l in sliced if l not in batch]
+        self.batch = [l for l in sliced if l in batch]
+        # Slice order of the results: loop indices first, then batch indices.
+        self.sliced = self.loop + self.batch
+        self.dims = {}
+        for ix, sh in zip(indices, shapes):
+            for l, s in zip(ix, sh):
+                self.dims[l] = int(s)
+        self.n_loop = int(np.prod([self.dims[l] for l in self.loop])) if self.loop else 1
+        self.n_batch = int(np.prod([self.dims[l] for l in self.batch])) if self.batch else 1
+        self.n_slices = self.n_loop * self.n_batch
+        self.use_graph = use_graph
+
+        # Leaves with the loop indices removed (their shapes for the plan).
+        loop_set = set(self.loop)
+        self.leaf_labels = [[l for l in ix if l not in loop_set] for ix in indices]
+        self.leaf_shapes = [[s for l, s in zip(ix, sh) if l not in loop_set]
+                            for ix, sh in zip(indices, shapes)]
+        self.net = DeviceNetwork(self.leaf_labels, self.leaf_shapes, path, self.batch,
+                                 dtype=self.dtype, precision=precision, ctx=self.ctx)
+        # Device copies of every leaf variant (one per value combination of
+        # the loop indices the leaf holds).
+        self._bufs: List[nat.DeviceBuffer] = []
+        self._variants: List[Dict[Tuple[int, ...], int]] = []
+        self._leaf_loop: List[List[str]] = []
+        for ix, sh, d in zip(indices, shapes, data):
+            held = [l for l in self.loop if l in ix]
+            self._leaf_loop.append(held)
+            var = {}
+            for vals in itertools.product(*[range(self.dims[l]) for l in held]):
+                _, _, arr = _slice_leaf(ix, sh, d, dict(zip(held, vals)))
+                arr = arr.astype(self.dtype)
+                buf = nat.DeviceBuffer(max(arr.nbytes, 16), self.ctx)
+                buf.upload(arr)
+                self._bufs.append(buf)
+                var[tuple(vals)] = buf.ptr
+            self._variants.append(var)
+        self.results = nat.DeviceBuffer(max(16, self.n_slices * self.dtype.itemsize), self.ctx)
+        self._sum = nat.DeviceBuffer(16, self.ctx)
+        self.flops_per_execution = self.net.flops
+
+    def _leaf_ptrs(self, loop_values: Dict[str, int]) -> List[int]:
+        return [var[tuple(loop_values[l] for l in held)]
+                for var, held in zip(self._variants, self._leaf_loop)]
+
+    def loop_values(self, i: int) -> Dict[str, int]:
+        vals, rem = {}, i
+        for l in reversed(self.loop):
+            vals[l] = rem % self.dims[l]
+            rem //= self.dims[l]
+        return vals
+
+    def run(self, loop_iterations: Optional[Sequence[int]] = None) -> None:
+        """Execute the given loop iterations (all by default); each writes its
+        n_batch per-slice results into the results vector."""
+        its = range(self.n_loop) if loop_iterations is None else loop_iterations
+        for i in its:
+            out = self.results.ptr + i * self.n_batch * self.dtype.itemsize
+            ptrs = self._leaf_ptrs(self.loop_values(i))
+            if self.use_graph:
+                self.net.launch(ptrs, out)
+            else:
+                self.net.execute(ptrs, out)
+
+    def slice_results(self, count: Optional[int] = None) -> np.ndarray:
+        n = self.n_slices if count is None else count
+        host = np.empty(n, dtype=self.dtype)
+        self.results.download(host)
+        return host
+
+    def reduce(self, count: Optional[int] = None) -> complex:
+        """FP64 compensated sum of the first `count` per-slice results."""
+        n = self.n_slices if count is None else count
+        nat.check(nat.lib().jet_hip_reduce_compensated(self.ctx.handle, self.code, n,
+                                                       self.results.ptr, self._sum.ptr, None))
+        host = np.empty(2, dtype=np.float64)
+        self._sum.download(host)
+        return complex(host[0], host[1])
+
+
+def choose_batch(indices, shapes, path, sliced: Sequence[str], budget_bytes: float,
+                 dtype=np.complex64, precision=None) -> List[str]:
+    """Largest set of sliced indices to batch whose plan fits in the budget:
+    start from all of them and move to the loop, one at a time, the index
+    whose removal shrinks the plan most."""
+    dims = {l: int(s) for ix, sh in zip(indices, shapes) for l, s in zip(ix, sh)}
+
+    def cost(batch):
+        loop = set(sliced) - set(batch)
+        li = [[l for l in ix if l not in loop] for ix in indices]
+        ls = [[s for l, s in zip(ix, sh) if l not in loop] for ix, sh in zip(indices, shapes)]
+        fl, ar, ws = estimate(li, ls, path, [l for l in sliced if l in batch], dtype, precision)
+        return ar + ws
+
+    batch = list(sliced)
+    while batch and cost(batch) > budget_bytes:
+        best = min(batch, key=lambda l: cost([x for x in batch if x != l]))
+        batch.remove(best)
+    return [l for l in sliced if l in batch]

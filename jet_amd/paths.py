@@ -1,0 +1,442 @@
+"""Contraction paths and slice selection for tensor networks.
+
+Jet itself has no path finder in C++: its Python layer samples random paths
+(python/jet/interpreter.py:533-618) and the paper benchmarks load cotengra
+paths from JSON.  This module provides
+
+  * greedy_path: greedy pairwise contraction order (at each step contract the
+    connected pair whose result minimises size(out) - size(a) - size(b)),
+    optionally randomised with Gumbel noise and repeated, keeping the path
+    with the fewest flops;
+  * path_stats: per-step flops / sizes in the PathInfo conventions
+    (PathInfo.hpp:157-224);
+  * choose_slices: greedy choice of the indices to slice
+    (TensorNetwork::SliceIndices, TensorNetwork.hpp:210-284): repeatedly the
+    index whose removal most reduces the largest intermediate, then the flops.
+
+Paths use SSA ids like PathInfo: leaves are 0..n-1 and step i creates n+i.
+"""
+from __future__ import annotations
+
+import heapq
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Set, Tuple
+
+import numpy as np
+
+Path = List[Tuple[int, int]]
+
+
+def _log2size(labels, dims) -> float:
+    return sum(math.log2(dims[l]) for l in labels)
+
+
+def greedy_path(indices: Sequence[Sequence[str]], dims: Dict[str, int],
+                open_indices: Sequence[str] = (), seed: Optional[int] = None,
+                temperature: float = 0.0) -> Path:
+    """One greedy contraction path (SSA ids)."""
+    rng = np.random.default_rng(seed)
+    n = len(indices)
+    open_set = set(open_indices)
+    alive: Dict[int, frozenset] = {i: frozenset(ix) for i, ix in enumerate(indices)}
+    where: Dict[str, Set[int]] = {}
+    for i, ix in alive.items():
+        for l in ix:
+            where.setdefault(l, set()).add(i)
+    size = {i: _log2size(ix, dims) for i, ix in alive.items()}
+
+    def result_labels(a: int, b: int) -> frozenset:
+        la, lb = alive[a], alive[b]
+        keep = set()
+        for l in la | lb:
+            others = len(where[l] - {a, b})
+            if l in open_set or others > 0 or not (l in la and l in lb):
+                keep.add(l)
+        return frozenset(keep)
+
+    def cost(a: int, b: int) -> float:
+        out = result_labels(a, b)
+        c = 2.0 ** _log2size(out, dims) - 2.0 ** size[a] - 2.0 ** size[b]
+        if temperature > 0:
+            c -= temperature * math.log(-math.log(rng.uniform(1e-12, 1.0))) * (
+                2.0 ** max(size[a], size[b]))
+        return c
+
+    heap: List[Tuple[float, int, int, int]] = []
+    counter = 0
+
+    def push_neighbours(a: int):
+        nonlocal counter
+        seen = set()
+        for l in alive[a]:
+            for b in where[l]:
+                if b != a and b not in seen:
+                    seen.add(b)
+                    x, y = (a, b) if a < b else (b, a)
+                    heapq.heappush(heap, (cost(x, y), counter, x, y))
+                    counter += 1
+
+    for i in list(alive):
+        push_neighbours(i)
+
+    path: Path = []
+    next_id = n
+    while len(alive) > 1:
+        pair = None
+        while heap:
+            _, _, a, b = heapq.heappop(heap)
+            if a in alive and b in alive:
+                pair = (a, b)
+                break
+        if pair is None:
+            # Disconnected components: contract the two smallest tensors.
+            a, b = sorted(alive, key=lambda i: (size[i], i))[:2]
+            pair = (a, b)
+        a, b = pair
+        out = result_labels(a, b)
+        for l in alive[a] | alive[b]:
+            where[l].discard(a)
+            where[l].discard(b)
+            if l in out:
+                where[l].add(next_id)
+        del alive[a], alive[b]
+        alive[next_id] = out
+        size[next_id] = _log2size(out, dims)
+        path.append((a, b))
+        push_neighbours(next_id)
+        next_id += 1
+    return path
+
+
+@dataclass
+class StepStats:
+    out: List[frozenset]          # labels of each step's result
+    contracted: List[frozenset]   # labels summed at each step
+    log2_size: np.ndarray         # log2 |result| per step
+    log2_flops: np.ndarray        # log2 (|result| * prod contracted) per step
+    total_flops: float            # PathInfo convention: sum |out| * 2 * prod contracted
+    max_log2_size: float
+
+
+def path_stats(indices: Sequence[Sequence[str]], dims: Dict[str, int], path: Path,
+               sliced: Sequence[str] = (), open_indices: Sequence[str] = ()) -> StepStats:
+    """Per-step results of contracting along `path` with `sliced` indices
+    removed (PathInfo.hpp:262-297: result = symmetric difference)."""
+    sl = set(sliced)
+    nodes: List[frozenset] = [frozenset(l for l in ix if l not in sl) for ix in indices]
+    outs, cons, ls, lf = [], [], [], []
+    total = 0
+    for (a, b) in path:
+        la, lb = nodes[a], nodes[b]
+        c = (la & lb) - set(open_indices)
+        o = (la | lb) - c
+        nodes.append(o)
+        outs.append(o)
+        cons.append(c)
+        size_o = math.prod(dims[l] for l in o)
+        size_c = math.prod(dims[l] for l in c)
+        ls.append(math.log2(size_o))
+        lf.append(math.log2(size_o * size_c))
+        total += size_o * 2 * size_c
+    ls = np.array(ls)
+    lf = np.array(lf)
+    total = float(total)
+    return StepStats(outs, cons, ls, lf, total, float(ls.max()) if len(ls) else 0.0)
+
+
+def best_greedy_path(indices, dims, trials: int = 16, seed: int = 0,
+                     temperature: float = 0.02, open_indices=()) -> Path:
+    """Deterministic greedy plus `trials` randomised runs; fewest flops wins."""
+    best, best_cost = None, None
+    for t in range(trials + 1):
+        p = greedy_path(indices, dims, open_indices, seed=seed + t,
+                        temperature=0.0 if t == 0 else temperature)
+        st = path_stats(indices, dims, p, open_indices=open_indices)
+        key = (st.total_flops, st.max_log2_size)
+        if best_cost is None or key < best_cost:
+            best, best_cost = p, key
+    return best
+
+
+def choose_slices(indices, dims, path: Path, n_slices: int,
+                  candidates: Optional[Sequence[str]] = None) -> List[str]:
+    """Greedy slice selection: each round removes the index that minimises
+    (largest intermediate, total flops) of the per-slice contraction."""
+    st = path_stats(indices, dims, path)
+    labels = sorted(candidates if candidates is not None else
+                    {l for ix in indices for l in ix})
+    col = {l: j for j, l in enumerate(labels)}
+    nstep = len(path)
+    O = np.zeros((nstep, len(labels)), dtype=np.float64)   # label in result
+    F = np.zeros((nstep, len(labels)), dtype=np.float64)   # label in result or contracted
+    for s in range(nstep):
+        for l in st.out[s]:
+            if l in col:
+                O[s, col[l]] = math.log2(dims[l])
+                F[s, col[l]] = math.log2(dims[l])
+        for l in st.contracted[s]:
+            if l in col:
+                F[s, col[l]] = math.log2(dims[l])
+    ls = st.log2_size.copy()
+    lf = st.log2_flops.copy()
+    chosen: List[str] = []
+    avail = np.ones(len(labels), dtype=bool)
+    for _ in range(n_slices):
+        new_max = np.max(ls[:, None] - O, axis=0)
+        new_flops = np.sum(2.0 ** (lf[:, None] - F), axis=0)
+        key = np.where(avail, new_max * 1e6 + np.log2(new_flops + 1.0), np.inf)
+        j = int(np.argmin(key))
+        chosen.append(labels[j])
+        avail[j] = False
+        ls = ls - O[:, j]
+        lf = lf - F[:, j]
+    return chosen
+
+
+def subtree_leaves(n_leaves: int, path: Path) -> List[Set[int]]:
+    """Leaf ids below each SSA node."""
+    leaves: List[Set[int]] = [{i} for i in range(n_leaves)]
+    for (a, b) in path:
+        leaves.append(leaves[a] | leaves[b])
+    return leaves
+
+
+def elimination_path(indices: Sequence[Sequence[str]], dims: Dict[str, int],
+                     seed: Optional[int] = None, open_indices: Sequence[str] = (),
+                     noise: float = 0.0) -> Path:
+    """Contraction path from a greedy index-elimination order on the line
+    graph (indices adjacent when they share a tensor): repeatedly eliminate
+    the index whose neighbourhood is smallest (min-degree, log2-weighted;
+    `noise` adds random tie-breaking), contracting the tensors that hold it.
+    For circuit networks this follows the treewidth heuristics of variable
+    elimination and gives far cheaper paths than pairwise greedy."""
+    rng = np.random.default_rng(seed)
+    open_set = set(open_indices)
+    w = {l: math.log2(dims[l]) for ix in indices for l in ix}
+    adj: Dict[str, Set[str]] = {l: set() for l in w}
+    for ix in indices:
+        s = set(ix)
+        for a in s:
+            adj[a] |= s - {a}
+    alive: Dict[int, Set[str]] = {i: set(ix) for i, ix in enumerate(indices)}
+    where: Dict[str, Set[int]] = {l: set() for l in w}
+    for i, ix in alive.items():
+        for l in ix:
+            where[l].add(i)
+    path: Path = []
+    next_id = len(indices)
+
+    def contract(a: int, b: int) -> int:
+        nonlocal next_id
+        la, lb = alive.pop(a), alive.pop(b)
+        out = set()
+        for l in la | lb:
+            where[l].discard(a)
+            where[l].discard(b)
+            if l in open_set or where[l] or not (l in la and l in lb):
+                out.add(l)
+        for l in out:
+            where[l].add(next_id)
+        alive[next_id] = out
+        path.append((a, b))
+        next_id += 1
+        return next_id - 1
+
+    def degree(v: str) -> float:
+        return sum(w[u] for u in adj[v])
+
+    heap = [(degree(v) + (noise * rng.random() if noise else 0.0), v) for v in adj
+            if v not in open_set]
+    heapq.heapify(heap)
+    eliminated: Set[str] = set()
+    while heap:
+        d, v = heapq.heappop(heap)
+        if v in eliminated:
+            continue
+        cur = degree(v) + (noise * rng.random() if noise else 0.0)
+        if cur > d + 1e-9:
+            heapq.heappush(heap, (cur, v))
+            continue
+        eliminated.add(v)
+        ts = sorted(where[v], key=lambda i: (len(alive[i]), i))
+        while len(ts) > 1:
+            a, b = ts[0], ts[1]
+            c = contract(a, b)
+            ts = sorted([c] + ts[2:], key=lambda i: (len(alive[i]), i))
+        nbrs = adj.pop(v)
+        for u in nbrs:
+            adj[u].discard(v)
+            adj[u] |= nbrs - {u}
+        for u in nbrs:
+            if u not in eliminated and u not in open_set:
+                heapq.heappush(heap, (degree(u) + (noise * rng.random() if noise else 0.0), u))
+    rest = sorted(alive, key=lambda i: (len(alive[i]), i))
+    while len(rest) > 1:
+        c = contract(rest[0], rest[1])
+        rest = sorted([c] + rest[2:], key=lambda i: (len(alive[i]), i))
+    return path
+
+
+def _absorb_small(indices: Sequence[Sequence[str]], open_set: Set[str]):
+    """Contract every tensor of rank <= 2 into the neighbour it shares the
+    most indices with (gates on one wire, input and output states)."""
+    alive: Dict[int, Set[str]] = {i: set(ix) for i, ix in enumerate(indices)}
+    where: Dict[str, Set[int]] = {}
+    for i, ix in alive.items():
+        for l in ix:
+            where.setdefault(l, set()).add(i)
+    path: Path = []
+    nid = len(indices)
+    changed = True
+    while changed:
+        changed = False
+        for i in sorted(alive, key=lambda i: (len(alive[i]), i)):
+            if i not in alive or len(alive[i]) > 2:
+                continue
+            nbrs = {j for l in alive[i] for j in where[l] if j != i}
+            if not nbrs:
+                continue
+            j = max(sorted(nbrs), key=lambda j: (len(alive[i] & alive[j]), -len(alive[j])))
+            la, lb = alive.pop(i), alive.pop(j)
+            out = set()
+            for l in la | lb:
+                where[l].discard(i)
+                where[l].discard(j)
+                if l in open_set or where[l] or not (l in la and l in lb):
+                    out.add(l)
+            for l in out:
+                where[l].add(nid)
+            alive[nid] = out
+            path.append((i, j))
+            nid += 1
+            changed = True
+    return path, alive, where, nid
+
+
+def _minfill_order(tensors: Sequence[Sequence[str]], rng, noise: float) -> List[str]:
+    """Greedy min-fill elimination order of the line graph (ties by degree,
+    randomised by `noise`)."""
+    adj: Dict[str, Set[str]] = {}
+    for ix in tensors:
+        s = set(ix)
+        for a in s:
+            adj.setdefault(a, set()).update(s - {a})
+
+    def fill(v):
+        nb = list(adj[v])
+        f = 0
+        for i in range(len(nb)):
+            ai = adj[nb[i]]
+            for j in range(i + 1, len(nb)):
+                if nb[j] not in ai:
+                    f += 1
+        return f
+
+    score = {v: fill(v) for v in adj}
+    order: List[str] = []
+    alive = set(adj)
+    while alive:
+        best = min(sorted(alive), key=lambda v: (score[v] + (noise * rng.random() if noise else 0.0),
+                                                 len(adj[v])))
+        nb = adj.pop(best)
+        order.append(best)
+        alive.discard(best)
+        for u in nb:
+            adj[u].discard(best)
+            adj[u] |= nb - {u}
+        touched = set(nb)
+        for u in nb:
+            touched |= adj[u]
+        for u in touched:
+            if u in alive:
+                score[u] = fill(u)
+    return order
+
+
+def minfill_path(indices: Sequence[Sequence[str]], dims: Dict[str, int], seed: int = 0,
+                 noise: float = 0.0, open_indices: Sequence[str] = ()) -> Path:
+    """Absorb rank<=2 tensors, then contract along a min-fill elimination
+    order of the remaining network's indices."""
+    open_set = set(open_indices)
+    path, alive, where, nid = _absorb_small(indices, open_set)
+    rng = np.random.default_rng(seed)
+    ids = sorted(alive)
+    order = _minfill_order([sorted(alive[i]) for i in ids], rng, noise)
+
+    def contract(a, b):
+        nonlocal nid
+        la, lb = alive.pop(a), alive.pop(b)
+        out = set()
+        for l in la | lb:
+            where[l].discard(a)
+            where[l].discard(b)
+            if l in open_set or where[l] or not (l in la and l in lb):
+                out.add(l)
+        for l in out:
+            where[l].add(nid)
+        alive[nid] = out
+        path.append((a, b))
+        nid += 1
+        return nid - 1
+
+    for v in order:
+        if v in open_set:
+            continue
+        ts = sorted(where.get(v, ()), key=lambda i: (len(alive[i]), i))
+        while len(ts) > 1:
+            c = contract(ts[0], ts[1])
+            ts = sorted([c] + ts[2:], key=lambda i: (len(alive[i]), i))
+    rest = sorted(alive, key=lambda i: (len(alive[i]), i))
+    while len(rest) > 1:
+        c = contract(rest[0], rest[1])
+        rest = sorted([c] + rest[2:], key=lambda i: (len(alive[i]), i))
+    return path
+
+
+def best_minfill_path(indices, dims, trials: int = 16, seed: int = 0,
+                      open_indices=()) -> Path:
+    """Deterministic min-fill plus randomised trials; fewest flops wins."""
+    best, best_key = None, None
+    for t in range(trials):
+        p = minfill_path(indices, dims, seed=seed + t,
+                         noise=0.0 if t == 0 else 0.3 + 0.1 * (t % 10),
+                         open_indices=open_indices)
+        st = path_stats(indices, dims, p, open_indices=open_indices)
+        key = (st.total_flops, st.max_log2_size)
+        if best_key is None or key < best_key:
+            best, best_key = p, key
+    return best
+
+
+def choose_slices_for_flops(indices, dims, path: Path, n_slices: int,
+                            max_log2_size: Optional[float] = None) -> List[str]:
+    """Greedy slice selection minimising the per-slice flops, while keeping
+    the largest per-slice intermediate reachable below max_log2_size."""
+    st = path_stats(indices, dims, path)
+    labels = sorted({l for ix in indices for l in ix})
+    col = {l: j for j, l in enumerate(labels)}
+    ns = len(path)
+    O = np.zeros((ns, len(labels)))
+    F = np.zeros((ns, len(labels)))
+    for s in range(ns):
+        for l in st.out[s]:
+            O[s, col[l]] = math.log2(dims[l])
+            F[s, col[l]] = math.log2(dims[l])
+        for l in st.contracted[s]:
+            F[s, col[l]] = math.log2(dims[l])
+    ls, lf = st.log2_size.copy(), st.log2_flops.copy()
+    chosen: List[str] = []
+    avail = np.ones(len(labels), dtype=bool)
+    for i in range(n_slices):
+        new_max = np.max(ls[:, None] - O, axis=0)
+        key = np.log2(np.sum(2.0 ** (lf[:, None] - F), axis=0))
+        if max_log2_size is not None:
+            key = key + 100.0 * np.maximum(0.0, new_max - (max_log2_size + (n_slices - i - 1)))
+        key[~avail] = np.inf
+        j = int(np.argmin(key))
+        chosen.append(labels[j])
+        avail[j] = False
+        ls = ls - O[:, j]
+        lf = lf - F[:, j]
+    return chosen
