@@ -1,8 +1,20 @@
 """Benchmark of the MI355X contraction path (one JSON line on stdout).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1]
 
 Workloads:
+  c2  (default) one amplitude <b|C|0> of a seeded 7x7-qubit grid random
+      circuit of depth 14 ({sqrt X, sqrt Y, sqrt W} on every qubit, then
+      fSim(pi/2, pi/6) on one of four alternating nearest-neighbour edge
+      patterns per cycle), complex64, contracted along a min-fill path
+      sliced on 12 bond-2 indices (4096 slices).  The slice loop runs on the
+      GPU in batches: 2^b slices per execution (b sliced indices kept as
+      batch modes), executions over the other sliced indices.  A step is one
+      execution (2^b slices); the timed region runs --steps executions in
+      split-BF16 3M, then the same in plain FP32 MFMA; one execution in
+      complex128 on FP64 MFMA is the FP64 reference of the relative error.
+      Each execution's per-slice amplitudes are summed by the FP64
+      compensated reduction.
   c1  single large pairwise contraction: complex64 A, B of rank 28 (all
       extents 2, 2^28 elements each), 14 shared indices at seeded random
       positions in both index orders, re/im ~ N(0, 1/2); one
@@ -11,8 +23,8 @@ Workloads:
       value); the plain FP32-MFMA path is timed beside it, and complex128 on
       FP64 MFMA gives the FP64 reference the error is measured against.
 
-value = algorithmic TFLOP/s (8*M*N*K real flops per contraction) with the
-operands resident in HBM when the timed region starts.  The timed region is
+value = algorithmic TFLOP/s (8*M*N*K real flops per executed GEMM) with the
+operands resident in HBM when the timed region starts; slices/s beside it.  The timed region is
 bracketed by HIP events on the library's stream and a full synchronization.
 """
 from __future__ import annotations
@@ -255,12 +267,199 @@ def c1_cpu_baseline(host, threads):
                       f"{dt:.1f} s"}
 
 
+# ---------------------------------------------------------------------------
+# C2: sliced 7x7 depth-14 amplitude, batched slice loop
+# ---------------------------------------------------------------------------
+
+C2_GRID = (7, 7, 14)
+C2_SEED = 2026
+C2_SLICES = 12
+
+
+def c2_problem():
+    from jet_amd import circuits, paths
+    plan = paths.load_or_make_grid_plan(*C2_GRID, C2_SEED, C2_SLICES)
+    circ = circuits.random_grid_circuit(*C2_GRID, seed=C2_SEED)
+    net = circuits.amplitude_network(circ, dtype=np.complex64)
+    return plan, circ, net
+
+
+def run_c2(args):
+    from jet_amd import _native as nat
+    from jet_amd.network import SlicedContraction
+
+    ctx = nat.default_context()
+    t0 = time.perf_counter()
+    plan, circ, net = c2_problem()
+    path, sliced = plan["path"], plan["sliced"]
+    batch = sliced[len(sliced) - args.batch_log2:] if args.batch_log2 else []
+    log(f"c2: plan ready ({time.perf_counter() - t0:.1f}s): {len(net)} tensors, "
+        f"{len(path)} steps, per-slice PathInfo flops {plan['per_slice_pathinfo_flops']:.3e}, "
+        f"batch {len(batch)} of {len(sliced)} sliced indices")
+
+    def build(prec, dtype):
+        data = [d.astype(dtype) for d in net.data]
+        return SlicedContraction(net.indices, net.shapes, data, path, sliced, batch=batch,
+                                 dtype=dtype, precision=prec, ctx=ctx, use_graph=True)
+
+    def timed(sc, iters, warmup, profile):
+        sc.run(range(warmup))
+        ctx.synchronize()
+        if profile:
+            nat.Profiler.reset()
+            nat.Profiler.enable(True)
+        e0, e1 = nat.Event(ctx), nat.Event(ctx)
+        e0.record()
+        sc.run(range(warmup, warmup + iters))
+        e1.record()
+        ctx.synchronize()
+        ms = e0.elapsed_ms(e1)
+        prof = None
+        if profile:
+            prof = nat.Profiler.read()
+            nat.Profiler.enable(False)
+        return ms, prof
+
+    steps, warm = args.steps, args.warmup
+    sc16 = build("bf16x3", np.complex64)
+    n_batch = sc16.n_batch
+    exec_flops = sc16.flops_per_execution
+    log(f"c2: bf16x3 network: {exec_flops:.3e} flops/execution, arena "
+        f"{sc16.net.arena_bytes / 2**30:.1f} GiB, workspace {sc16.net.workspace_bytes / 2**30:.1f} GiB")
+    ms16, prof16 = timed(sc16, steps, warm, True)
+    res16 = sc16.slice_results((warm + steps) * n_batch)
+    sum16 = sc16.reduce((warm + steps) * n_batch)
+    log(f"c2: bf16x3 {ms16 / steps:.1f} ms/execution, {exec_flops * steps / ms16 / 1e9:.1f} TFLOP/s")
+    del sc16
+
+    steps32 = max(1, args.steps32)
+    sc32 = build("fp32", np.complex64)
+    ms32, prof32 = timed(sc32, steps32, 0, True)
+    res32 = sc32.slice_results(steps32 * n_batch)
+    log(f"c2: fp32 {ms32 / steps32:.1f} ms/execution, "
+        f"{exec_flops * steps32 / ms32 / 1e9:.1f} TFLOP/s")
+    del sc32
+
+    # FP64 reference of the first execution's slices (complex128, FP64 MFMA).
+    sc64 = build("fp64", np.complex128)
+    ms64, _ = timed(sc64, 1, 0, False)
+    res64 = sc64.slice_results(n_batch)
+    sum64 = sc64.reduce(n_batch)
+    del sc64
+    n_ref = n_batch
+    err16 = float(np.linalg.norm(res16[:n_ref] - res64) / np.linalg.norm(res64))
+    err32 = float(np.linalg.norm(res32[:n_ref] - res64) / np.linalg.norm(res64))
+    amp16 = complex(np.sum(res16[:n_ref].astype(np.complex128)))
+    amp_err16 = abs(amp16 - sum64) / abs(sum64)
+    log(f"c2: fp64 {ms64:.1f} ms/execution; rel err per-slice bf16x3 {err16:.2e} fp32 {err32:.2e}; "
+        f"partial-amplitude err {amp_err16:.2e}")
+
+    slices16 = steps * n_batch
+    tflops16 = exec_flops * steps / (ms16 * 1e-3)
+    tflops32 = exec_flops * steps32 / (ms32 * 1e-3)
+    sps16 = slices16 / (ms16 * 1e-3)
+    sps32 = steps32 * n_batch / (ms32 * 1e-3)
+    dom = max(prof16.items(), key=lambda kv: kv[1]["ms"])
+    dname, dk = dom
+    bound = "mfma" if dk["flops"] > 0 else "hbm"
+    if bound == "mfma":
+        achieved = dk["flops"] / (dk["ms"] * 1e-3)
+        peak = PEAK["bf16"] if "bf16" in dname else PEAK["fp32"]
+        unit, scale = "TFLOP/s", 1e12
+    else:
+        achieved = dk["bytes"] / (dk["ms"] * 1e-3)
+        peak, unit, scale = PEAK["hbm"], "GB/s", 1e9
+    return {
+        "metric": METRIC,
+        "value": round(tflops16 / 1e12, 3),
+        "unit": "TFLOP/s",
+        "ms_per_step": round(ms16 / steps, 3),
+        "dtype": "c64 (split-bf16x3 MFMA on MFMA-bound steps, fp32 elsewhere)",
+        "config": {"workload": "c2: one amplitude of a seeded 7x7-qubit grid random circuit, "
+                               "depth 14, complex64, min-fill path sliced on 12 bond-2 indices "
+                               "(4096 slices); a step is one batched execution of "
+                               f"{n_batch} slices",
+                   "qubits": 49, "depth": 14, "sliced_indices": len(sliced),
+                   "slices_total": 4096, "slices_per_step": n_batch,
+                   "tensors": len(net), "path_steps": len(path),
+                   "per_slice_pathinfo_flops": plan["per_slice_pathinfo_flops"],
+                   "per_slice_max_log2_size": plan["per_slice_max_log2_size"]},
+        "slices_per_s": round(sps16, 4),
+        "fp32_mfma": {"ms_per_step": round(ms32 / steps32, 3), "tflops": round(tflops32 / 1e12, 3),
+                      "slices_per_s": round(sps32, 4)},
+        "speedup_bf16x3_vs_fp32": round(tflops16 / tflops32, 3),
+        "projected_amplitude_s": {"bf16x3": round(4096 / sps16, 1), "fp32": round(4096 / sps32, 1)},
+        "rel_err_vs_fp64": {"bf16x3_per_slice": err16, "fp32_per_slice": err32,
+                            "bf16x3_partial_amplitude": amp_err16,
+                            "slices_compared": n_ref},
+        "fp64_mfma": {"ms_per_step": round(ms64, 3)},
+        "kernels_bf16x3": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
+                           for k, v in prof16.items()},
+        "kernels_fp32": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
+                         for k, v in prof32.items()},
+        "roofline": {"kernel": dname, "bound": bound, "achieved": round(achieved / scale, 3),
+                     "peak": peak / scale, "unit": unit, "frac": round(achieved / peak, 4),
+                     "traffic": None},
+    }, (plan, net)
+
+
+def c2_cpu_baseline(host, threads, budget_s=20.0):
+    """Reference CPU path (QFlex transposes + OpenBLAS cgemm, one
+    ContractTensors per step, Tensor.hpp:708-752) on the most expensive
+    contraction steps of one slice, with their real index orders and shapes
+    and constant operand data, until ~budget_s of CPU work."""
+    import oracle
+    from jet_amd import paths
+    plan, net = host
+    sl = set(plan["sliced"])
+    labels = [[l for l in ix if l not in sl] for ix in net.indices]
+    steps = []
+    for (a, b) in plan["path"]:
+        la, lb = labels[a], labels[b]
+        left = [x for x in la if x not in lb]
+        right = [x for x in lb if x not in la]
+        common = [x for x in la if x in lb]
+        labels.append(left + right)
+        steps.append((la, lb, 8.0 * 2 ** (len(left) + len(right) + len(common))))
+    order = sorted(range(len(steps)), key=lambda i: -steps[i][2])
+    oracle.load_blas(threads)
+    done_flops, t_used, ran = 0.0, 0.0, 0
+    est_rate = 5e11
+    for i in order:
+        la, lb, fl = steps[i]
+        if t_used + fl / est_rate > budget_s and ran > 0:
+            continue
+        a = np.full(2 ** len(la), 0.5 + 0.25j, np.complex64)
+        b = np.full(2 ** len(lb), 0.5 - 0.25j, np.complex64)
+        t0 = time.perf_counter()
+        oracle.baseline_contract(la, [2] * len(la), a, lb, [2] * len(lb), b, threads)
+        dt = time.perf_counter() - t0
+        t_used += dt
+        done_flops += fl
+        ran += 1
+        est_rate = done_flops / t_used
+        if t_used > budget_s:
+            break
+    rate = done_flops / t_used
+    per_slice = 4.0 * plan["per_slice_pathinfo_flops"]
+    return {"value": round(rate / 1e12, 4), "unit": "TFLOP/s", "cores": threads,
+            "kind": "port", "slices_per_s": rate / per_slice,
+            "sample": f"the {ran} most expensive contraction steps of one c2 slice that fit "
+                      f"{budget_s:.0f} s (QFlex transposes + OpenBLAS cblas_cgemm per step), "
+                      f"{done_flops:.3e} flops in {t_used:.1f} s; slices/s = that rate / the "
+                      f"per-slice flops"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c1", choices=["c1"])
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--batch-log2", type=int, default=2,
+                    help="c2: sliced indices batched per execution (2^b slices per step)")
+    ap.add_argument("--steps32", type=int, default=1,
+                    help="c2: executions timed in plain FP32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--quick", action="store_true",
@@ -272,14 +471,18 @@ def main():
               "not run.", file=sys.stderr)
         sys.exit(2)
 
-    result, host = run_c1(args)
-    if args.quick:
-        print(json.dumps(result), flush=True)
-        return
+    if args.workload == "c1":
+        result, host = run_c1(args)
+        if args.quick:
+            print(json.dumps(result), flush=True)
+            return
+    else:
+        result, host = run_c2(args)
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        result["cpu_baseline"] = c1_cpu_baseline(host, threads)
+        result["cpu_baseline"] = (c1_cpu_baseline(host, threads) if args.workload == "c1"
+                                  else c2_cpu_baseline(host, threads))
     else:
         result["cpu_baseline"] = None
     result.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

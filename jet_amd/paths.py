@@ -440,3 +440,67 @@ def choose_slices_for_flops(indices, dims, path: Path, n_slices: int,
         ls = ls - O[:, j]
         lf = lf - F[:, j]
     return chosen
+
+
+def grid_circuit_plan(rows: int, cols: int, depth: int, seed: int, n_slices: int,
+                      trials: int = 16, max_log2_size: float = 29.0) -> dict:
+    """Contraction path and sliced indices for one amplitude of a seeded grid
+    random circuit (jet_amd.circuits): best min-fill path of `trials`, then
+    `n_slices` indices chosen for per-slice flops."""
+    from . import circuits
+    circ = circuits.random_grid_circuit(rows, cols, depth, seed=seed)
+    net = circuits.amplitude_network(circ)
+    dims = {l: 2 for ix in net.indices for l in ix}
+    path = best_minfill_path(net.indices, dims, trials=trials, seed=seed)
+    sliced = choose_slices_for_flops(net.indices, dims, path, n_slices, max_log2_size)
+    full = path_stats(net.indices, dims, path)
+    per = path_stats(net.indices, dims, path, sliced=sliced)
+    return {"rows": rows, "cols": cols, "depth": depth, "seed": seed, "trials": trials,
+            "n_tensors": len(net), "path": [list(p) for p in path], "sliced": sliced,
+            "pathinfo_flops": full.total_flops, "max_log2_size": full.max_log2_size,
+            "per_slice_pathinfo_flops": per.total_flops,
+            "per_slice_max_log2_size": per.max_log2_size}
+
+
+def load_or_make_grid_plan(rows, cols, depth, seed, n_slices, trials=16, max_log2_size=29.0,
+                           directory: Optional[str] = None) -> dict:
+    """Plan from jet_amd/data (written by `python -m jet_amd.paths`) when one
+    with the same parameters exists, else computed here."""
+    import json
+    import os
+    directory = directory or os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+    name = os.path.join(directory, f"grid_{rows}x{cols}x{depth}_seed{seed}_s{n_slices}.json")
+    if os.path.exists(name):
+        with open(name) as f:
+            plan = json.load(f)
+        if (plan["rows"], plan["cols"], plan["depth"], plan["seed"], len(plan["sliced"])) == \
+                (rows, cols, depth, seed, n_slices):
+            plan["path"] = [tuple(p) for p in plan["path"]]
+            return plan
+    plan = grid_circuit_plan(rows, cols, depth, seed, n_slices, trials, max_log2_size)
+    plan["path"] = [tuple(p) for p in plan["path"]]
+    return plan
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import os
+    ap = argparse.ArgumentParser(description="write a grid-circuit contraction plan to jet_amd/data")
+    ap.add_argument("--rows", type=int, default=7)
+    ap.add_argument("--cols", type=int, default=7)
+    ap.add_argument("--depth", type=int, default=14)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--slices", type=int, default=12)
+    ap.add_argument("--trials", type=int, default=64)
+    ap.add_argument("--max-log2-size", type=float, default=29.0)
+    a = ap.parse_args()
+    plan = grid_circuit_plan(a.rows, a.cols, a.depth, a.seed, a.slices, a.trials, a.max_log2_size)
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+    os.makedirs(d, exist_ok=True)
+    out = os.path.join(d, f"grid_{a.rows}x{a.cols}x{a.depth}_seed{a.seed}_s{a.slices}.json")
+    with open(out, "w") as f:
+        json.dump(plan, f)
+    print(out, "pathinfo flops %.3e max %d, per slice %.3e max %d" % (
+        plan["pathinfo_flops"], plan["max_log2_size"], plan["per_slice_pathinfo_flops"],
+        plan["per_slice_max_log2_size"]))
