@@ -64,7 +64,11 @@ typedef enum {
                                  hi+lo BF16, three BF16 MFMA products per real
                                  product, FP32 accumulation.  C64 only.  Steps
                                  the plan finds HBM-bound run on FP32 MFMA.   */
-    JET_HIP_PREC_FP64 = 3     /* 4M complex GEMM on FP64 MFMA (C128 only)      */
+    JET_HIP_PREC_FP64 = 3,    /* 4M complex GEMM on FP64 MFMA (C128 only)      */
+    JET_HIP_PREC_BF16X3_4M = 4 /* as BF16X3, but the complex product in 4M form
+                                  (12 BF16 MFMA per real-product triple; twice
+                                  the MFMA work per operand byte of 3M and no
+                                  Im = T3 - T1 - T2 cancellation)              */
 } jet_hip_precision_t;
 
 typedef struct jet_hip_ctx_s *jet_hip_ctx_t;

@@ -39,6 +39,7 @@ int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
         // Split-BF16 pays only where the step is bound by MFMA throughput
         // (every dimension large enough to reuse operands from LDS).
         if (mode == 1 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x3;
+        if (mode == 3 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x4m;
         return kGemmF32Mfma;
     }
     if (K > 256 && batch * M * N <= 65536) return kGemmSmallRed;
@@ -55,7 +56,7 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
     JH_REQUIRE(ra >= 0 && rb >= 0 && rc >= 0 && ra <= kMaxRank && rb <= kMaxRank &&
                    rc <= kMaxRank,
                "Tensor rank must be between 0 and 64.");
-    JH_REQUIRE(!(mode == 1 && elem_bytes != 8), "Split-BF16 needs complex64 data.");
+    JH_REQUIRE(!((mode == 1 || mode == 3) && elem_bytes != 8), "Split-BF16 needs complex64 data.");
     JH_REQUIRE(!(mode == 2 && elem_bytes != 16), "FP64 MFMA needs complex128 data.");
     JH_REQUIRE(!(mode == 0 && elem_bytes != 8), "FP32 MFMA needs complex64 data.");
 
@@ -174,13 +175,14 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
         d.off_ct = off;
         off = align256(off + eb * (size_t)d.size_c);
     }
-    if (d.gemm_kernel == kGemmBf16x3) {
+    if (d.gemm_kernel == kGemmBf16x3 || d.gemm_kernel == kGemmBf16x4m) {
+        const size_t planes = d.gemm_kernel == kGemmBf16x3 ? 6 : 4;
         d.split_a_plane = split_plane_elems(d.batch, d.M, d.K);
         d.split_b_plane = split_plane_elems(d.batch, d.N, d.K);
         d.off_sa = off;
-        off = align256(off + 6 * 2 * (size_t)d.split_a_plane);
+        off = align256(off + planes * 2 * (size_t)d.split_a_plane);
         d.off_sb = off;
-        off = align256(off + 6 * 2 * (size_t)d.split_b_plane);
+        off = align256(off + planes * 2 * (size_t)d.split_b_plane);
     }
     d.workspace = off;
     return d;

@@ -58,6 +58,30 @@ __device__ __forceinline__ f64x4 mfma4(double a, double b, f64x4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Map the linear work-group id to an output tile.  Work-groups are dispatched
+// round-robin over the 8 XCDs (each with its own L2), so first give every XCD
+// a contiguous range of tile indices, then order tiles in groups of
+// kGroupM tile-rows so that the work-groups resident on one XCD share A and B
+// panels in its L2.
+constexpr int kXcds = 8;
+constexpr int kGroupM = 8;
+
+__device__ __forceinline__ void tile_coords(int64_t id, int64_t tiles_m, int64_t tiles_n,
+                                            int64_t &tm, int64_t &tn)
+{
+    const int64_t total = tiles_m * tiles_n;
+    const int64_t q = total / kXcds, r = total % kXcds;
+    const int64_t x = id % kXcds, k = id / kXcds;
+    const int64_t j = x * q + (x < r ? x : r) + k;
+    const int64_t per_group = int64_t(kGroupM) * tiles_n;
+    const int64_t group = j / per_group;
+    const int64_t first_m = group * kGroupM;
+    const int64_t gsz = (tiles_m - first_m) < kGroupM ? (tiles_m - first_m) : kGroupM;
+    const int64_t w = j % per_group;
+    tm = first_m + w % gsz;
+    tn = w / gsz;
+}
+
 // ---------------------------------------------------------------------------
 // 4M complex GEMM on FP32 / FP64 MFMA.
 // Block tile BM x BN complex outputs, BK complex k per LDS stage, 4 waves in
@@ -86,7 +110,9 @@ __global__ __launch_bounds__(256) void cgemm_embed_kernel(GemmArgs g)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
     const int64_t bz = blockIdx.z;
-    const int64_t m0 = int64_t(blockIdx.x) * BM, n0 = int64_t(blockIdx.y) * BN;
+    int64_t tm, tn;
+    tile_coords(blockIdx.x, (g.M + BM - 1) / BM, (g.N + BN - 1) / BN, tm, tn);
+    const int64_t m0 = tm * BM, n0 = tn * BN;
     const C2 *__restrict__ A = static_cast<const C2 *>(g.A) + bz * g.sA;
     const C2 *__restrict__ B = static_cast<const C2 *>(g.B) + bz * g.sB;
     R *__restrict__ C = reinterpret_cast<R *>(static_cast<C2 *>(g.C) + bz * g.sC);
@@ -292,7 +318,8 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(
         const int64_t off = (b * dst.rows_p + r) * dst.k_p + kq * 8;
 #pragma unroll
         for (int p = 0; p < 6; p++)
-            *reinterpret_cast<bf16x8 *>(base + p * dst.plane_elems + off) = o[p];
+            if (p < dst.planes)
+                *reinterpret_cast<bf16x8 *>(base + p * dst.plane_elems + off) = o[p];
     }
 }
 
@@ -307,24 +334,27 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(
 // into y, leaving x = Re C and y = Im C.
 // ---------------------------------------------------------------------------
 template <int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(256) void cgemm_bf16x3_kernel(
+__global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x3_kernel(
     SplitOperand a, SplitOperand b, float2 *__restrict__ Cout, int64_t M,
     int64_t N, int64_t ldc, int64_t sC, int a_bcast, int b_bcast)
 {
+    constexpr int NT = 64 * WM * WN;
     constexpr int KT = 32;
     constexpr int LROW = KT + 8; // bf16 per LDS row (16-byte pad)
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
-    constexpr int A_CH = 2 * BM * (KT / 8) / 256; // 16-byte chunks per thread
-    constexpr int B_CH = 2 * BN * (KT / 8) / 256;
-    static_assert(WM * WN == 4 && A_CH * 256 == 2 * BM * 4 && B_CH * 256 == 2 * BN * 4, "tile");
+    constexpr int A_CH = 2 * BM * (KT / 8) / NT; // 16-byte chunks per thread
+    constexpr int B_CH = 2 * BN * (KT / 8) / NT;
+    static_assert(A_CH * NT == 2 * BM * 4 && B_CH * NT == 2 * BN * 4, "tile");
     __shared__ bf16x8 As[2][2 * BM * LROW / 8];
     __shared__ bf16x8 Bs[2][2 * BN * LROW / 8];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
     const int64_t bz = blockIdx.z;
-    const int64_t m0 = int64_t(blockIdx.x) * BM, n0 = int64_t(blockIdx.y) * BN;
+    int64_t tm, tn;
+    tile_coords(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    const int64_t m0 = tm * BM, n0 = tn * BN;
     const __bf16 *Ab = static_cast<const __bf16 *>(a.base) +
                        (a_bcast ? 0 : bz * a.rows_p * a.k_p);
     const __bf16 *Bb = static_cast<const __bf16 *>(b.base) +
@@ -346,7 +376,7 @@ __global__ __launch_bounds__(256) void cgemm_bf16x3_kernel(
         const int ph = s / nk;
 #pragma unroll
         for (int i = 0; i < A_CH; i++) {
-            const int c = tid + i * 256;
+            const int c = tid + i * NT;
             const int pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
             const int64_t m = m0 + r;
             ra[i] = (m < a.rows_p)
@@ -356,7 +386,7 @@ __global__ __launch_bounds__(256) void cgemm_bf16x3_kernel(
         }
 #pragma unroll
         for (int i = 0; i < B_CH; i++) {
-            const int c = tid + i * 256;
+            const int c = tid + i * NT;
             const int pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
             const int64_t n = n0 + r;
             rb[i] = (n < b.rows_p)
@@ -368,13 +398,13 @@ __global__ __launch_bounds__(256) void cgemm_bf16x3_kernel(
     auto swrite = [&](int buf) {
 #pragma unroll
         for (int i = 0; i < A_CH; i++) {
-            const int c = tid + i * 256;
+            const int c = tid + i * NT;
             const int pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
             As[buf][(pl * BM + r) * (LROW / 8) + kc] = ra[i];
         }
 #pragma unroll
         for (int i = 0; i < B_CH; i++) {
-            const int c = tid + i * 256;
+            const int c = tid + i * NT;
             const int pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
             Bs[buf][(pl * BN + r) * (LROW / 8) + kc] = rb[i];
         }
@@ -399,39 +429,38 @@ __global__ __launch_bounds__(256) void cgemm_bf16x3_kernel(
                 }
         }
         if (s + 1 < nstages) gload(s + 1);
-        bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
-#pragma unroll
-        for (int fm = 0; fm < FM; fm++) {
-            const int r = wm * WTM + fm * 16 + j;
-            ah[fm] = As[buf][r * (LROW / 8) + q];
-            al[fm] = As[buf][(BM + r) * (LROW / 8) + q];
-        }
+        // B fragments stay in registers for the whole stage; A fragments are
+        // loaded per row block to keep the register footprint small.
+        bf16x8 bh[FN], bl[FN];
 #pragma unroll
         for (int fn = 0; fn < FN; fn++) {
             const int r = wn * WTN + fn * 16 + j;
             bh[fn] = Bs[buf][r * (LROW / 8) + q];
             bl[fn] = Bs[buf][(BN + r) * (LROW / 8) + q];
         }
-#define JH_BF16_PRODUCT(P)                                                                  \
+#define JH_BF16_STAGE(P)                                                                    \
     _Pragma("unroll") for (int fm = 0; fm < FM; fm++)                                       \
     {                                                                                       \
+        const int r = wm * WTM + fm * 16 + j;                                               \
+        const bf16x8 ah = As[buf][r * (LROW / 8) + q];                                      \
+        const bf16x8 al = As[buf][(BM + r) * (LROW / 8) + q];                               \
         _Pragma("unroll") for (int fn = 0; fn < FN; fn++)                                   \
         {                                                                                   \
             acc[P][fm][fn] =                                                                \
-                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[fm], bh[fn], acc[P][fm][fn], 0, 0, 0); \
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[fn], acc[P][fm][fn], 0, 0, 0); \
             acc[P][fm][fn] =                                                                \
-                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[fm], bl[fn], acc[P][fm][fn], 0, 0, 0); \
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[fn], acc[P][fm][fn], 0, 0, 0); \
             acc[P][fm][fn] =                                                                \
-                __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[fm], bh[fn], acc[P][fm][fn], 0, 0, 0); \
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[fn], acc[P][fm][fn], 0, 0, 0); \
         }                                                                                   \
     }
         if (ph == 0) {
-            JH_BF16_PRODUCT(0)
+            JH_BF16_STAGE(0)
         }
         else {
-            JH_BF16_PRODUCT(1)
+            JH_BF16_STAGE(1)
         }
-#undef JH_BF16_PRODUCT
+#undef JH_BF16_STAGE
         if (s + 1 < nstages) swrite(buf ^ 1);
         __syncthreads();
     }
@@ -447,6 +476,140 @@ __global__ __launch_bounds__(256) void cgemm_bf16x3_kernel(
                 const int64_t row = m0 + wm * WTM + fm * 16 + 4 * q + r;
                 if (row < M && col < N)
                     C[row * ldc + col] = float2{acc[0][fm][fn][r], acc[1][fm][fn][r]};
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4M split-BF16 complex GEMM.
+// Re C = Ar.Br - Ai.Bi and Im C = Ar.Bi + Ai.Br, each real product split into
+// the three BF16 terms Xh.Yh + Xh.Yl + Xl.Yh (12 v_mfma_f32_16x16x32_bf16 per
+// 16x16 block and 32 k).  Every k tile stages the four planes (re hi/lo, im
+// hi/lo) of both operands once, so each byte loaded into LDS feeds twice as
+// many MFMAs as in the 3M form, and no T3 - T1 - T2 cancellation occurs.
+// One LDS buffer per block (two blocks per CU overlap loads and MFMAs).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bf16x8 neg_bf16x8(bf16x8 v)
+{
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
+    SplitOperand a, SplitOperand b, float2 *__restrict__ Cout, int64_t M,
+    int64_t N, int64_t ldc, int64_t sC, int a_bcast, int b_bcast)
+{
+    constexpr int NT = 64 * WM * WN;
+    constexpr int KT = 32;
+    constexpr int LROW8 = (KT + 8) / 8; // bf16x8 per LDS row (16-byte pad)
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    constexpr int A_CH = 4 * BM * (KT / 8) / NT;
+    constexpr int B_CH = 4 * BN * (KT / 8) / NT;
+    static_assert(A_CH * NT == 4 * BM * 4 && B_CH * NT == 4 * BN * 4, "tile");
+    __shared__ bf16x8 As[4 * BM * LROW8];
+    __shared__ bf16x8 Bs[4 * BN * LROW8];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int64_t bz = blockIdx.z;
+    int64_t tm, tn;
+    tile_coords(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+    const __bf16 *Ab = static_cast<const __bf16 *>(a.base) +
+                       (a_bcast ? 0 : bz * a.rows_p * a.k_p);
+    const __bf16 *Bb = static_cast<const __bf16 *>(b.base) +
+                       (b_bcast ? 0 : bz * b.rows_p * b.k_p);
+
+    f32x4 cr[FM][FN], ci[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int jj = 0; jj < FN; jj++) {
+            cr[i][jj] = f32x4{0, 0, 0, 0};
+            ci[i][jj] = f32x4{0, 0, 0, 0};
+        }
+
+    const bf16x8 zero = {};
+    const int nk = int(a.k_p / KT);
+    const int q = lane >> 4, j = lane & 15;
+    for (int kt = 0; kt < nk; kt++) {
+        const int64_t k0 = int64_t(kt) * KT;
+        // global -> LDS: chunk c = (plane, row, 16-byte column)
+#pragma unroll
+        for (int i = 0; i < A_CH; i++) {
+            const int c = tid + i * NT;
+            const int pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+            const int64_t m = m0 + r;
+            As[(pl * BM + r) * LROW8 + kc] =
+                (m < a.rows_p) ? *reinterpret_cast<const bf16x8 *>(
+                                     Ab + pl * a.plane_elems + m * a.k_p + k0 + kc * 8)
+                               : zero;
+        }
+#pragma unroll
+        for (int i = 0; i < B_CH; i++) {
+            const int c = tid + i * NT;
+            const int pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+            const int64_t n = n0 + r;
+            Bs[(pl * BN + r) * LROW8 + kc] =
+                (n < b.rows_p) ? *reinterpret_cast<const bf16x8 *>(
+                                     Bb + pl * b.plane_elems + n * b.k_p + k0 + kc * 8)
+                               : zero;
+        }
+        __syncthreads();
+        bf16x8 brh[FN], brl[FN], bih[FN], bil[FN];
+#pragma unroll
+        for (int fn = 0; fn < FN; fn++) {
+            const int r = wn * WTN + fn * 16 + j;
+            brh[fn] = Bs[(0 * BN + r) * LROW8 + q];
+            brl[fn] = Bs[(1 * BN + r) * LROW8 + q];
+            bih[fn] = Bs[(2 * BN + r) * LROW8 + q];
+            bil[fn] = Bs[(3 * BN + r) * LROW8 + q];
+        }
+#pragma unroll
+        for (int fm = 0; fm < FM; fm++) {
+            const int r = wm * WTM + fm * 16 + j;
+            const bf16x8 arh = As[(0 * BM + r) * LROW8 + q];
+            const bf16x8 arl = As[(1 * BM + r) * LROW8 + q];
+            const bf16x8 aih = As[(2 * BM + r) * LROW8 + q];
+            const bf16x8 ail = As[(3 * BM + r) * LROW8 + q];
+            const bf16x8 naih = neg_bf16x8(aih), nail = neg_bf16x8(ail);
+#pragma unroll
+            for (int fn = 0; fn < FN; fn++) {
+                f32x4 x = cr[fm][fn], y = ci[fm][fn];
+                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brh[fn], x, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bih[fn], y, 0, 0, 0);
+                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brl[fn], x, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bil[fn], y, 0, 0, 0);
+                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, brh[fn], x, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, bih[fn], y, 0, 0, 0);
+                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bih[fn], x, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh[fn], y, 0, 0, 0);
+                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bil[fn], x, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl[fn], y, 0, 0, 0);
+                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, bih[fn], x, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh[fn], y, 0, 0, 0);
+                cr[fm][fn] = x;
+                ci[fm][fn] = y;
+            }
+        }
+        __syncthreads();
+    }
+
+    float2 *C = Cout + bz * sC;
+#pragma unroll
+    for (int fm = 0; fm < FM; fm++) {
+#pragma unroll
+        for (int fn = 0; fn < FN; fn++) {
+            const int64_t col = n0 + wn * WTN + fn * 16 + j;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = m0 + wm * WTM + fm * 16 + 4 * q + r;
+                if (row < M && col < N) C[row * ldc + col] = float2{cr[fm][fn][r], ci[fm][fn][r]};
             }
         }
     }
@@ -503,7 +666,8 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
         JH_REQUIRE(elem_bytes == 8, "FP32 MFMA kernel needs complex64 data.");
         constexpr int BM = 128, BN = 128;
         JH_REQUIRE(ceil_div(g.N, BN) <= 65535, "N too large.");
-        dim3 grid((unsigned)ceil_div(g.M, BM), (unsigned)ceil_div(g.N, BN), (unsigned)g.batch);
+        JH_REQUIRE(ceil_div(g.M, BM) * ceil_div(g.N, BN) <= 0x7fffffff, "Too many tiles.");
+        dim3 grid((unsigned)(ceil_div(g.M, BM) * ceil_div(g.N, BN)), 1, (unsigned)g.batch);
         prof::Scope ps(prof::kGemmF32, stream, gemm_flops(g), gemm_bytes(g, 8));
         hipLaunchKernelGGL((cgemm_embed_kernel<float, BM, BN, 16, 2, 2>), grid, dim3(256), 0,
                            stream, g);
@@ -512,7 +676,8 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
         JH_REQUIRE(elem_bytes == 16, "FP64 MFMA kernel needs complex128 data.");
         constexpr int BM = 128, BN = 64;
         JH_REQUIRE(ceil_div(g.N, BN) <= 65535, "N too large.");
-        dim3 grid((unsigned)ceil_div(g.M, BM), (unsigned)ceil_div(g.N, BN), (unsigned)g.batch);
+        JH_REQUIRE(ceil_div(g.M, BM) * ceil_div(g.N, BN) <= 0x7fffffff, "Too many tiles.");
+        dim3 grid((unsigned)(ceil_div(g.M, BM) * ceil_div(g.N, BN)), 1, (unsigned)g.batch);
         prof::Scope ps(prof::kGemmF64, stream, gemm_flops(g), gemm_bytes(g, 16));
         hipLaunchKernelGGL((cgemm_embed_kernel<double, BM, BN, 8, 2, 2>), grid, dim3(256), 0,
                            stream, g);
@@ -532,11 +697,30 @@ void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
     const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
     // reads 8 bytes and writes 6 bf16 per (padded) element
     prof::Scope ps(prof::kSplitBf16, stream, 0.0,
-                   8.0 * double(batch) * double(rows) * double(k) + 12.0 * double(dst.batch) *
-                                                                      double(dst.rows_p) *
-                                                                      double(dst.k_p));
+                   8.0 * double(batch) * double(rows) * double(k) +
+                       2.0 * dst.planes * double(dst.batch) * double(dst.rows_p) * double(dst.k_p));
     hipLaunchKernelGGL(split_bf16_kernel, dim3(grid), dim3(256), 0, stream,
                        static_cast<const float2 *>(src), rows, k, ld, s, dst);
+    JH_HIP(hipGetLastError());
+}
+
+void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b,
+                         void *C, int64_t M, int64_t N, int64_t ldc,
+                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
+                         hipStream_t stream)
+{
+    constexpr int BM = 128, BN = 128, WM = 2, WN = 2;
+    JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
+    JH_REQUIRE(a.planes >= 4 && b.planes >= 4, "Split operands need four planes.");
+    JH_REQUIRE(batch <= 65535, "Batch count exceeds 65535.");
+    JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
+    dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);
+    const double K = double(a.k_p);
+    prof::Scope ps(prof::kGemmBf16x4m, stream, 8.0 * double(batch) * double(M) * double(N) * K,
+                   8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
+    hipLaunchKernelGGL((cgemm_bf16x4m_kernel<BM, BN, WM, WN>), grid, dim3(64 * WM * WN), 0,
+                       stream, a, b, static_cast<float2 *>(C), M, N, ldc, sC, a_bcast ? 1 : 0,
+                       b_bcast ? 1 : 0);
     JH_HIP(hipGetLastError());
 }
 
@@ -545,15 +729,16 @@ void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                         hipStream_t stream)
 {
-    constexpr int BM = 128, BN = 128;
+    constexpr int BM = 128, BN = 128, WM = 2, WN = 2;
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
-    JH_REQUIRE(batch <= 65535 && ceil_div(N, BN) <= 65535, "Grid too large.");
-    dim3 grid((unsigned)ceil_div(M, BM), (unsigned)ceil_div(N, BN), (unsigned)batch);
+    JH_REQUIRE(batch <= 65535, "Batch count exceeds 65535.");
+    JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
+    dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);
     const double K = double(a.k_p);
     prof::Scope ps(prof::kGemmBf16x3, stream, 8.0 * double(batch) * double(M) * double(N) * K,
                    8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
-    hipLaunchKernelGGL((cgemm_bf16x3_kernel<BM, BN, 2, 2>), grid, dim3(256), 0, stream, a, b,
-                       static_cast<float2 *>(C), M, N, ldc, sC, a_bcast ? 1 : 0,
+    hipLaunchKernelGGL((cgemm_bf16x3_kernel<BM, BN, WM, WN>), grid, dim3(64 * WM * WN), 0, stream,
+                       a, b, static_cast<float2 *>(C), M, N, ldc, sC, a_bcast ? 1 : 0,
                        b_bcast ? 1 : 0);
     JH_HIP(hipGetLastError());
 }

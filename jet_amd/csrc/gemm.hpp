@@ -30,17 +30,20 @@ enum GemmKernel {
     kGemmF32Mfma = 2,   // 4M complex on v_mfma_f32_16x16x4_f32
     kGemmF64Mfma = 3,   // 4M complex on v_mfma_f64_16x16x4_f64
     kGemmBf16x3 = 4,    // 3M complex, split-BF16, v_mfma_f32_16x16x32_bf16
+    kGemmBf16x4m = 5,   // 4M complex, split-BF16, v_mfma_f32_16x16x32_bf16
 };
 
 // Rows and k of the split-BF16 operand planes are padded to these multiples.
 constexpr int kSplitRowPad = 16;
 constexpr int kSplitKPad = 32;
 
-// Split-BF16 operand: 6 planes (re hi/lo, im hi/lo, re+im hi/lo), each
-// [batch][rows_p][k_p] bf16, plane p at offset p * plane_elems.
+// Split-BF16 operand: planes (re hi/lo, im hi/lo[, re+im hi/lo]), each
+// [batch][rows_p][k_p] bf16, plane p at offset p * plane_elems.  The 3M
+// kernel reads all six planes, the 4M kernel the first four.
 struct SplitOperand {
     void *base;
     int64_t rows_p, k_p, batch, plane_elems;
+    int planes;
 };
 
 inline int64_t split_plane_elems(int64_t batch, int64_t rows, int64_t k)
@@ -51,7 +54,8 @@ inline int64_t split_plane_elems(int64_t batch, int64_t rows, int64_t k)
 }
 
 // Which kernel the plan uses for a given shape and precision mode
-// (0 = fp32/complex64, 1 = bf16x3/complex64, 2 = fp64/complex128).
+// (0 = fp32/complex64, 1 = split-bf16 3M/complex64, 2 = fp64/complex128,
+//  3 = split-bf16 4M/complex64).
 int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
                        int64_t batch);
 
@@ -62,6 +66,10 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
 void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
                        int64_t s, int64_t batch, const SplitOperand &dst,
                        hipStream_t stream);
+void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b,
+                         void *C, int64_t M, int64_t N, int64_t ldc,
+                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
+                         hipStream_t stream);
 void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b,
                         void *C, int64_t M, int64_t N, int64_t ldc,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,

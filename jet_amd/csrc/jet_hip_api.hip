@@ -97,6 +97,8 @@ int mode_of(jet_hip_dtype_t dt, jet_hip_precision_t p)
         return 1;
     case JET_HIP_PREC_FP64:
         return 2;
+    case JET_HIP_PREC_BF16X3_4M:
+        return 3;
     }
     JH_FAIL(JET_HIP_ERROR_INVALID_VALUE, "Unknown precision.");
 }
@@ -141,14 +143,18 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         Bt = ws + d.off_bt;
     }
     void *Ct = d.need_c ? static_cast<void *>(ws + d.off_ct) : c;
-    if (d.gemm_kernel == kGemmBf16x3) {
+    if (d.gemm_kernel == kGemmBf16x3 || d.gemm_kernel == kGemmBf16x4m) {
+        const int planes = d.gemm_kernel == kGemmBf16x3 ? 6 : 4;
         SplitOperand sa{ws + d.off_sa, ceil_div(d.M, kSplitRowPad) * kSplitRowPad,
-                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_a_plane};
+                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_a_plane, planes};
         SplitOperand sb{ws + d.off_sb, ceil_div(d.N, kSplitRowPad) * kSplitRowPad,
-                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_b_plane};
+                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_b_plane, planes};
         launch_split_bf16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
         launch_split_bf16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
-        launch_gemm_bf16x3(sa, sb, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
+        if (planes == 6)
+            launch_gemm_bf16x3(sa, sb, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
+        else
+            launch_gemm_bf16x4m(sa, sb, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
     }
     else {
         GemmArgs g;

@@ -27,6 +27,7 @@ enum Kernel {
     kReduce,
     kConvert,
     kGemmSkinny,
+    kGemmBf16x4m,
     kNumKernels
 };
 

@@ -19,7 +19,7 @@ from hiputil import contract, permute, rand_c, rel_err, say
 pytestmark = pytest.mark.gpu
 
 TOL = {nat.PREC_FP32: 1e-5, nat.PREC_BF16X3: 1e-5, nat.PREC_FP64: 1e-12,
-       nat.PREC_DEFAULT: 1e-5}
+       nat.PREC_DEFAULT: 1e-5, nat.PREC_BF16X3_4M: 1e-5}
 
 
 # --------------------------------------------------------------------------
@@ -133,6 +133,7 @@ def _random_problem(rng, max_rank=6, dims=(1, 2, 3, 4)):
 
 @pytest.mark.parametrize("prec,dtype", [(nat.PREC_FP32, np.complex64),
                                         (nat.PREC_BF16X3, np.complex64),
+                                        (nat.PREC_BF16X3_4M, np.complex64),
                                         (nat.PREC_FP64, np.complex128)])
 def test_contract_random_small(ctx, prec, dtype):
     rng = np.random.default_rng(100 + prec)
@@ -154,6 +155,9 @@ def test_contract_random_small(ctx, prec, dtype):
     (nat.PREC_FP32, np.complex64, (33, 17, 16)),
     (nat.PREC_BF16X3, np.complex64, (512, 512, 512)),
     (nat.PREC_BF16X3, np.complex64, (300, 200, 170)),
+    (nat.PREC_BF16X3_4M, np.complex64, (512, 512, 512)),
+    (nat.PREC_BF16X3_4M, np.complex64, (300, 200, 170)),
+    (nat.PREC_BF16X3_4M, np.complex64, (1000, 260, 1100)),
     (nat.PREC_FP64, np.complex128, (256, 128, 96)),
     (nat.PREC_FP64, np.complex128, (130, 70, 50)),
     (nat.PREC_FP32, np.complex64, (8, 4, 4000)),
@@ -174,6 +178,8 @@ def test_gemm_kernels(ctx, prec, dtype, shape):
     assert err < TOL[prec], (shape, info, err)
     if prec == nat.PREC_BF16X3 and min(shape) >= 128:
         assert info["kernel"] == "bf16x3_mfma"
+    if prec == nat.PREC_BF16X3_4M and min(shape) >= 128:
+        assert info["kernel"] == "bf16x3_4m_mfma"
 
 
 def test_batch_modes(ctx):
