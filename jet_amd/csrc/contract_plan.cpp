@@ -42,6 +42,7 @@ int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
         if (mode == 3 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x4m;
         return kGemmF32Mfma;
     }
+    if (K >= 4 * kSplitKChunk && batch * M * N <= 65536) return kGemmSplitK;
     if (K > 256 && batch * M * N <= 65536) return kGemmSmallRed;
     return kGemmSmall;
 }
@@ -183,6 +184,10 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
         off = align256(off + planes * 2 * (size_t)d.split_a_plane);
         d.off_sb = off;
         off = align256(off + planes * 2 * (size_t)d.split_b_plane);
+    }
+    if (d.gemm_kernel == kGemmSplitK) {
+        d.off_partials = off;
+        off = align256(off + 16 * (size_t)splitk_partials(d.batch * d.M * d.N, d.K));
     }
     d.workspace = off;
     return d;

@@ -33,7 +33,7 @@ struct ContractionDesc {
     PermutePlan pa, pb, pc;
     int gemm_kernel = kGemmSmall;
     // Workspace layout (bytes, 256-aligned offsets).
-    size_t off_at = 0, off_bt = 0, off_ct = 0, off_sa = 0, off_sb = 0;
+    size_t off_at = 0, off_bt = 0, off_ct = 0, off_sa = 0, off_sb = 0, off_partials = 0;
     int64_t split_a_plane = 0, split_b_plane = 0;
     size_t workspace = 0;
     double flops = 0; // 8 * batch * M * N * K

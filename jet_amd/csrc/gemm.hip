@@ -488,7 +488,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x3_kernel(
 // 16x16 block and 32 k).  Every k tile stages the four planes (re hi/lo, im
 // hi/lo) of both operands once, so each byte loaded into LDS feeds twice as
 // many MFMAs as in the 3M form, and no T3 - T1 - T2 cancellation occurs.
-// One LDS buffer per block (two blocks per CU overlap loads and MFMAs).
+// Double-buffered LDS with the next k tile prefetched into registers.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bf16x8 neg_bf16x8(bf16x8 v)
 {
@@ -505,14 +505,16 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
 {
     constexpr int NT = 64 * WM * WN;
     constexpr int KT = 32;
-    constexpr int LROW8 = (KT + 8) / 8; // bf16x8 per LDS row (16-byte pad)
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
-    constexpr int A_CH = 4 * BM * (KT / 8) / NT;
+    constexpr int A_CH = 4 * BM * (KT / 8) / NT; // 16-byte chunks per thread
     constexpr int B_CH = 4 * BN * (KT / 8) / NT;
     static_assert(A_CH * NT == 4 * BM * 4 && B_CH * NT == 4 * BN * 4, "tile");
-    __shared__ bf16x8 As[4 * BM * LROW8];
-    __shared__ bf16x8 Bs[4 * BN * LROW8];
+    // LDS rows of 32 bf16 (four 16-byte chunks) without padding; chunk kc of
+    // row r is stored at kc ^ ((r >> 2) & 3), which spreads the 16 rows read
+    // by one MFMA operand fetch over all 64 banks.
+    __shared__ bf16x8 As[2][4 * BM * 4];
+    __shared__ bf16x8 Bs[2][4 * BN * 4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
@@ -534,49 +536,70 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
             ci[i][jj] = f32x4{0, 0, 0, 0};
         }
 
+    bf16x8 ra[A_CH], rb[B_CH];
     const bf16x8 zero = {};
-    const int nk = int(a.k_p / KT);
-    const int q = lane >> 4, j = lane & 15;
-    for (int kt = 0; kt < nk; kt++) {
+    auto gload = [&](int kt) {
         const int64_t k0 = int64_t(kt) * KT;
-        // global -> LDS: chunk c = (plane, row, 16-byte column)
 #pragma unroll
         for (int i = 0; i < A_CH; i++) {
             const int c = tid + i * NT;
             const int pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
             const int64_t m = m0 + r;
-            As[(pl * BM + r) * LROW8 + kc] =
-                (m < a.rows_p) ? *reinterpret_cast<const bf16x8 *>(
-                                     Ab + pl * a.plane_elems + m * a.k_p + k0 + kc * 8)
-                               : zero;
+            ra[i] = (m < a.rows_p) ? *reinterpret_cast<const bf16x8 *>(
+                                         Ab + pl * a.plane_elems + m * a.k_p + k0 + kc * 8)
+                                   : zero;
         }
 #pragma unroll
         for (int i = 0; i < B_CH; i++) {
             const int c = tid + i * NT;
             const int pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
             const int64_t n = n0 + r;
-            Bs[(pl * BN + r) * LROW8 + kc] =
-                (n < b.rows_p) ? *reinterpret_cast<const bf16x8 *>(
-                                     Bb + pl * b.plane_elems + n * b.k_p + k0 + kc * 8)
-                               : zero;
+            rb[i] = (n < b.rows_p) ? *reinterpret_cast<const bf16x8 *>(
+                                         Bb + pl * b.plane_elems + n * b.k_p + k0 + kc * 8)
+                                   : zero;
         }
-        __syncthreads();
+    };
+    auto swrite = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < A_CH; i++) {
+            const int c = tid + i * NT;
+            const int pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+            As[buf][(pl * BM + r) * 4 + (kc ^ ((r >> 2) & 3))] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < B_CH; i++) {
+            const int c = tid + i * NT;
+            const int pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+            Bs[buf][(pl * BN + r) * 4 + (kc ^ ((r >> 2) & 3))] = rb[i];
+        }
+    };
+
+    const int nk = int(a.k_p / KT);
+    const int q = lane >> 4, j = lane & 15;
+    gload(0);
+    swrite(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload(kt + 1);
         bf16x8 brh[FN], brl[FN], bih[FN], bil[FN];
 #pragma unroll
         for (int fn = 0; fn < FN; fn++) {
             const int r = wn * WTN + fn * 16 + j;
-            brh[fn] = Bs[(0 * BN + r) * LROW8 + q];
-            brl[fn] = Bs[(1 * BN + r) * LROW8 + q];
-            bih[fn] = Bs[(2 * BN + r) * LROW8 + q];
-            bil[fn] = Bs[(3 * BN + r) * LROW8 + q];
+            const int sw = q ^ ((r >> 2) & 3);
+            brh[fn] = Bs[buf][(0 * BN + r) * 4 + sw];
+            brl[fn] = Bs[buf][(1 * BN + r) * 4 + sw];
+            bih[fn] = Bs[buf][(2 * BN + r) * 4 + sw];
+            bil[fn] = Bs[buf][(3 * BN + r) * 4 + sw];
         }
 #pragma unroll
         for (int fm = 0; fm < FM; fm++) {
             const int r = wm * WTM + fm * 16 + j;
-            const bf16x8 arh = As[(0 * BM + r) * LROW8 + q];
-            const bf16x8 arl = As[(1 * BM + r) * LROW8 + q];
-            const bf16x8 aih = As[(2 * BM + r) * LROW8 + q];
-            const bf16x8 ail = As[(3 * BM + r) * LROW8 + q];
+            const int sw = q ^ ((r >> 2) & 3);
+            const bf16x8 arh = As[buf][(0 * BM + r) * 4 + sw];
+            const bf16x8 arl = As[buf][(1 * BM + r) * 4 + sw];
+            const bf16x8 aih = As[buf][(2 * BM + r) * 4 + sw];
+            const bf16x8 ail = As[buf][(3 * BM + r) * 4 + sw];
             const bf16x8 naih = neg_bf16x8(aih), nail = neg_bf16x8(ail);
 #pragma unroll
             for (int fn = 0; fn < FN; fn++) {
@@ -597,6 +620,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
                 ci[fm][fn] = y;
             }
         }
+        if (kt + 1 < nk) swrite(buf ^ 1);
         __syncthreads();
     }
 
@@ -630,6 +654,96 @@ double gemm_bytes(const GemmArgs &g, int elem_bytes)
     return elem_bytes * (a + b + c);
 }
 
+// Split-K dot products for few outputs and long k: block (o, chunk) sums
+// output o over one k chunk in FP64 and stores the partial; a second kernel
+// adds the partials of each output in FP64.  Consecutive blocks share the k
+// chunk, so operand rows reused by several outputs come from L2.
+template <class R>
+__global__ __launch_bounds__(256) void gemm_splitk_partial_kernel(GemmArgs g, double2 *partials,
+                                                                  int64_t chunk, int64_t nchunks)
+{
+    using C2 = typename Vec<R>::c;
+    __shared__ double red[2][256];
+    const int64_t o = blockIdx.x;
+    const int64_t c = blockIdx.y;
+    const int64_t n = o % g.N;
+    const int64_t rest = o / g.N;
+    const int64_t m = rest % g.M;
+    const int64_t b = rest / g.M;
+    const C2 *a = static_cast<const C2 *>(g.A) + b * g.sA + m * g.lda;
+    const C2 *bb = static_cast<const C2 *>(g.B) + b * g.sB + n * g.ldb;
+    const int64_t k0 = c * chunk;
+    const int64_t k1 = (k0 + chunk < g.K) ? k0 + chunk : g.K;
+    double sr = 0, si = 0;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+        const C2 x = a[k], y = bb[k];
+        sr = fma(double(x.x), double(y.x), sr);
+        sr = fma(-double(x.y), double(y.y), sr);
+        si = fma(double(x.x), double(y.y), si);
+        si = fma(double(x.y), double(y.x), si);
+    }
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[o * nchunks + c] = double2{red[0][0], red[1][0]};
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void gemm_splitk_final_kernel(GemmArgs g, const double2 *partials,
+                                                                int64_t nchunks)
+{
+    using C2 = typename Vec<R>::c;
+    __shared__ double red[2][256];
+    const int64_t o = blockIdx.x;
+    double sr = 0, si = 0;
+    for (int64_t c = threadIdx.x; c < nchunks; c += 256) {
+        const double2 v = partials[o * nchunks + c];
+        sr += v.x;
+        si += v.y;
+    }
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int64_t n = o % g.N;
+        const int64_t rest = o / g.N;
+        const int64_t m = rest % g.M;
+        const int64_t b = rest / g.M;
+        static_cast<C2 *>(g.C)[b * g.sC + m * g.ldc + n] = C2{R(red[0][0]), R(red[1][0])};
+    }
+}
+
+template <class R>
+void launch_splitk(const GemmArgs &g, void *partials, hipStream_t stream)
+{
+    const int64_t outputs = g.batch * g.M * g.N;
+    const int64_t chunk = splitk_chunk(g.K);
+    const int64_t nchunks = ceil_div(g.K, chunk);
+    JH_REQUIRE(partials != nullptr, "Split-K GEMM needs its partial-sum buffer.");
+    JH_REQUIRE(outputs <= 0x7fffffff && nchunks <= 65535, "Split-K grid too large.");
+    prof::Scope ps(prof::kGemmSplitK, stream, gemm_flops(g), gemm_bytes(g, int(sizeof(R) * 2)));
+    hipLaunchKernelGGL(gemm_splitk_partial_kernel<R>, dim3((unsigned)outputs, (unsigned)nchunks),
+                       dim3(256), 0, stream, g, static_cast<double2 *>(partials), chunk, nchunks);
+    JH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gemm_splitk_final_kernel<R>, dim3((unsigned)outputs), dim3(256), 0, stream,
+                       g, static_cast<const double2 *>(partials), nchunks);
+    JH_HIP(hipGetLastError());
+}
+
 template <class R>
 void launch_valu(int kernel, const GemmArgs &g, hipStream_t stream)
 {
@@ -651,9 +765,16 @@ void launch_valu(int kernel, const GemmArgs &g, hipStream_t stream)
 } // namespace
 
 void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
-                 hipStream_t stream)
+                 hipStream_t stream, void *partials)
 {
     if (g.M == 0 || g.N == 0 || g.batch == 0) return;
+    if (kernel == kGemmSplitK) {
+        if (elem_bytes == 8)
+            launch_splitk<float>(g, partials, stream);
+        else
+            launch_splitk<double>(g, partials, stream);
+        return;
+    }
     if (kernel == kGemmSmall || kernel == kGemmSmallRed) {
         if (elem_bytes == 8)
             launch_valu<float>(kernel, g, stream);

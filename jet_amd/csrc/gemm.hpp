@@ -31,7 +31,21 @@ enum GemmKernel {
     kGemmF64Mfma = 3,   // 4M complex on v_mfma_f64_16x16x4_f64
     kGemmBf16x3 = 4,    // 3M complex, split-BF16, v_mfma_f32_16x16x32_bf16
     kGemmBf16x4m = 5,   // 4M complex, split-BF16, v_mfma_f32_16x16x32_bf16
+    kGemmSplitK = 6,    // VALU, few outputs and long k: k split over blocks,
+                        // partial sums combined in FP64
 };
+
+// Split-K: k range per block, and the FP64 partial-sum buffer it needs.
+constexpr int64_t kSplitKChunk = 8192;
+inline int64_t splitk_chunk(int64_t K)
+{
+    const int64_t c = ceil_div(K, 65535);
+    return c > kSplitKChunk ? c : kSplitKChunk;
+}
+inline int64_t splitk_partials(int64_t outputs, int64_t K)
+{
+    return outputs * ceil_div(K, splitk_chunk(K));
+}
 
 // Rows and k of the split-BF16 operand planes are padded to these multiples.
 constexpr int kSplitRowPad = 16;
@@ -60,8 +74,10 @@ int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
                        int64_t batch);
 
 #if defined(__HIPCC__)
+// `partials` (splitk_partials(batch*M*N, K) pairs of doubles) is only used by
+// kGemmSplitK.
 void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
-                 hipStream_t stream);
+                 hipStream_t stream, void *partials = nullptr);
 // src: complex64 [batch][rows][k] with row stride ld and batch stride s.
 void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
                        int64_t s, int64_t batch, const SplitOperand &dst,

@@ -171,7 +171,7 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         g.sA = d.M * d.K;
         g.sB = d.N * d.K;
         g.sC = d.M * d.N;
-        launch_gemm(d.gemm_kernel, eb, g, s);
+        launch_gemm(d.gemm_kernel, eb, g, s, ws + d.off_partials);
     }
     if (d.need_c) launch_permute(d.pc, eb, Ct, c, s);
 }

@@ -26,7 +26,7 @@ enum Kernel {
     kFill,
     kReduce,
     kConvert,
-    kGemmSkinny,
+    kGemmSplitK,
     kGemmBf16x4m,
     kNumKernels
 };

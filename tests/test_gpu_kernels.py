@@ -161,6 +161,9 @@ def test_contract_random_small(ctx, prec, dtype):
     (nat.PREC_FP64, np.complex128, (256, 128, 96)),
     (nat.PREC_FP64, np.complex128, (130, 70, 50)),
     (nat.PREC_FP32, np.complex64, (8, 4, 4000)),
+    (nat.PREC_FP32, np.complex64, (1, 1, 1 << 20)),
+    (nat.PREC_FP32, np.complex64, (3, 5, 100000)),
+    (nat.PREC_FP64, np.complex128, (2, 1, 1 << 18)),
     (nat.PREC_FP64, np.complex128, (3, 5, 3000)),
 ])
 def test_gemm_kernels(ctx, prec, dtype, shape):
