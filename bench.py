@@ -465,7 +465,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--batch-log2", type=int, default=2,
                     help="c2: sliced indices batched per execution (2^b slices per step)")
-    ap.add_argument("--split", default="3m", choices=["3m", "4m"],
+    ap.add_argument("--split", default="4m", choices=["3m", "4m"],
                     help="complex form of the split-BF16 GEMM")
     ap.add_argument("--steps32", type=int, default=1,
                     help="c2: executions timed in plain FP32")
