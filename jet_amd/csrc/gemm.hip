@@ -315,11 +315,21 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(
             o[4][e] = h;
             o[5][e] = l;
         }
-        const int64_t off = (b * dst.rows_p + r) * dst.k_p + kq * 8;
+        const int64_t row = b * dst.rows_p + r;
+        if (dst.interleaved) {
+            // [row][k tile][plane][32]
+            const int64_t off = (row * (dst.k_p / 32) + kq / 4) * 128 + (kq % 4) * 8;
 #pragma unroll
-        for (int p = 0; p < 6; p++)
-            if (p < dst.planes)
-                *reinterpret_cast<bf16x8 *>(base + p * dst.plane_elems + off) = o[p];
+            for (int p = 0; p < 4; p++)
+                *reinterpret_cast<bf16x8 *>(base + off + p * 32) = o[p];
+        }
+        else {
+            const int64_t off = row * dst.k_p + kq * 8;
+#pragma unroll
+            for (int p = 0; p < 6; p++)
+                if (p < dst.planes)
+                    *reinterpret_cast<bf16x8 *>(base + p * dst.plane_elems + off) = o[p];
+        }
     }
 }
 
@@ -522,10 +532,11 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
     int64_t tm, tn;
     tile_coords(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
     const int64_t m0 = tm * BM, n0 = tn * BN;
+    // interleaved layout: a batch holds rows_p * k_p * 4 bf16
     const __bf16 *Ab = static_cast<const __bf16 *>(a.base) +
-                       (a_bcast ? 0 : bz * a.rows_p * a.k_p);
+                       (a_bcast ? 0 : bz * a.rows_p * a.k_p * 4);
     const __bf16 *Bb = static_cast<const __bf16 *>(b.base) +
-                       (b_bcast ? 0 : bz * b.rows_p * b.k_p);
+                       (b_bcast ? 0 : bz * b.rows_p * b.k_p * 4);
 
     f32x4 cr[FM][FN], ci[FM][FN];
 #pragma unroll
@@ -538,24 +549,27 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
 
     bf16x8 ra[A_CH], rb[B_CH];
     const bf16x8 zero = {};
+    // Chunk c of a tile: row c / 16, then 16 chunks of 16 bytes covering the
+    // 256 contiguous bytes of that row's k tile (plane (c % 16) / 4, column
+    // chunk c % 4).
+    const int64_t nkt_a = a.k_p / KT, nkt_b = b.k_p / KT;
     auto gload = [&](int kt) {
-        const int64_t k0 = int64_t(kt) * KT;
 #pragma unroll
         for (int i = 0; i < A_CH; i++) {
             const int c = tid + i * NT;
-            const int pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+            const int r = c / 16, w = c % 16;
             const int64_t m = m0 + r;
             ra[i] = (m < a.rows_p) ? *reinterpret_cast<const bf16x8 *>(
-                                         Ab + pl * a.plane_elems + m * a.k_p + k0 + kc * 8)
+                                         Ab + ((m * nkt_a + kt) * 16 + w) * 8)
                                    : zero;
         }
 #pragma unroll
         for (int i = 0; i < B_CH; i++) {
             const int c = tid + i * NT;
-            const int pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+            const int r = c / 16, w = c % 16;
             const int64_t n = n0 + r;
             rb[i] = (n < b.rows_p) ? *reinterpret_cast<const bf16x8 *>(
-                                         Bb + pl * b.plane_elems + n * b.k_p + k0 + kc * 8)
+                                         Bb + ((n * nkt_b + kt) * 16 + w) * 8)
                                    : zero;
         }
     };
@@ -563,13 +577,13 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
 #pragma unroll
         for (int i = 0; i < A_CH; i++) {
             const int c = tid + i * NT;
-            const int pl = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, kc = rem % 4;
+            const int r = c / 16, pl = (c % 16) / 4, kc = c % 4;
             As[buf][(pl * BM + r) * 4 + (kc ^ ((r >> 2) & 3))] = ra[i];
         }
 #pragma unroll
         for (int i = 0; i < B_CH; i++) {
             const int c = tid + i * NT;
-            const int pl = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, kc = rem % 4;
+            const int r = c / 16, pl = (c % 16) / 4, kc = c % 4;
             Bs[buf][(pl * BN + r) * 4 + (kc ^ ((r >> 2) & 3))] = rb[i];
         }
     };
@@ -832,7 +846,8 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b,
 {
     constexpr int BM = 128, BN = 128, WM = 2, WN = 2;
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
-    JH_REQUIRE(a.planes >= 4 && b.planes >= 4, "Split operands need four planes.");
+    JH_REQUIRE(a.planes == 4 && b.planes == 4 && a.interleaved && b.interleaved,
+               "The 4M kernel needs four interleaved planes.");
     JH_REQUIRE(batch <= 65535, "Batch count exceeds 65535.");
     JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
     dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);
@@ -852,6 +867,8 @@ void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b,
 {
     constexpr int BM = 128, BN = 128, WM = 2, WN = 2;
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
+    JH_REQUIRE(a.planes == 6 && b.planes == 6 && !a.interleaved && !b.interleaved,
+               "The 3M kernel needs six planar planes.");
     JH_REQUIRE(batch <= 65535, "Batch count exceeds 65535.");
     JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
     dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);

@@ -51,13 +51,17 @@ inline int64_t splitk_partials(int64_t outputs, int64_t K)
 constexpr int kSplitRowPad = 16;
 constexpr int kSplitKPad = 32;
 
-// Split-BF16 operand: planes (re hi/lo, im hi/lo[, re+im hi/lo]), each
-// [batch][rows_p][k_p] bf16, plane p at offset p * plane_elems.  The 3M
-// kernel reads all six planes, the 4M kernel the first four.
+// Split-BF16 operand: planes re hi/lo, im hi/lo[, re+im hi/lo].
+//  * planar (3M kernel, six planes): plane p is [batch][rows_p][k_p] bf16 at
+//    offset p * plane_elems;
+//  * interleaved (4M kernel, four planes): [batch][rows_p][k_p / 32][4][32]
+//    bf16, i.e. the four planes of one row and one 32-wide k tile form 256
+//    contiguous bytes (two full cache lines per row and k tile).
 struct SplitOperand {
     void *base;
     int64_t rows_p, k_p, batch, plane_elems;
     int planes;
+    int interleaved;
 };
 
 inline int64_t split_plane_elems(int64_t batch, int64_t rows, int64_t k)

@@ -146,9 +146,11 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
     if (d.gemm_kernel == kGemmBf16x3 || d.gemm_kernel == kGemmBf16x4m) {
         const int planes = d.gemm_kernel == kGemmBf16x3 ? 6 : 4;
         SplitOperand sa{ws + d.off_sa, ceil_div(d.M, kSplitRowPad) * kSplitRowPad,
-                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_a_plane, planes};
+                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_a_plane, planes,
+                        planes == 4 ? 1 : 0};
         SplitOperand sb{ws + d.off_sb, ceil_div(d.N, kSplitRowPad) * kSplitRowPad,
-                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_b_plane, planes};
+                        ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_b_plane, planes,
+                        planes == 4 ? 1 : 0};
         launch_split_bf16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
         launch_split_bf16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
         if (planes == 6)

@@ -309,3 +309,19 @@ def test_reduce_compensated(ctx):
     got = from_device(out, 2, np.float64)
     want = oracle.sum_compensated(x)
     assert abs(got[0] - want.real) < 1e-6 and abs(got[1] - want.imag) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [nat.PREC_BF16X3, nat.PREC_BF16X3_4M, nat.PREC_FP32])
+def test_batch_modes_mfma(ctx, prec):
+    """Batched MFMA GEMMs (batch modes in A, B and C) at tile-sized shapes."""
+    rng = np.random.default_rng(13)
+    S, M, K, N = 3, 160, 256, 144
+    a = rand_c(rng, S * M * K, np.complex64)
+    b = rand_c(rng, K * S * N, np.complex64)
+    say(f"batch modes mfma prec={prec}")
+    got, info = contract(ctx, a, [0, 1, 2], [S, M, K], b, [2, 0, 3], [K, S, N], [0, 1, 3],
+                         precision=prec)
+    want = np.einsum("smk,ksn->smn", a.reshape(S, M, K).astype(np.complex128),
+                     b.reshape(K, S, N).astype(np.complex128)).reshape(-1)
+    assert info["batch"] == S
+    assert rel_err(got, want) < 1e-5, (info, rel_err(got, want))
