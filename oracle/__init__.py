@@ -14,8 +14,10 @@ Layers:
   * pure-Python/numpy restatements of the host-side algebra used by the
     checks (index algebra, slicing, network contraction along a path).
 
-Parity status of each function (what pins it) is listed in the test files
-``tests/test_oracle_*.py``.
+Each function cites the reference lines it restates; ``tests/test_oracle.py``
+pins each one against the reference's fixtures and literal values, the
+reference's own permuter build, or closed forms (numpy transpose and einsum,
+exact rational sums, statevector simulation in tests/test_circuits_paths.py).
 """
 from __future__ import annotations
 

@@ -176,3 +176,50 @@ def test_sum_compensated_exact():
         got = oracle.sum_compensated(x)
         assert abs(got.real - float(exact_re)) <= 1e-9 * max(1.0, abs(float(exact_re)))
         assert abs(got.imag - float(exact_im)) <= 1e-9 * max(1.0, abs(float(exact_im)))
+
+
+def test_unravel_index_reference_literals():
+    # Test_Utilities.cpp:453-495.
+    assert oracle.unravel_index(0, []) == []
+    want = {0: [0, 0], 1: [0, 1], 2: [0, 2], 3: [1, 0], 4: [1, 1], 5: [1, 2]}
+    for i, w in want.items():
+        assert oracle.unravel_index(i, [2, 3]) == w
+    for idx, shape in ((6, [2, 3]), (0, [1, 0, 1])):
+        with pytest.raises(oracle.OracleError, match="Linear index does not fit in the shape."):
+            oracle.unravel_index(idx, shape)
+    # Brute force: every index of a mixed-radix shape, in numpy's C order.
+    shape = [3, 1, 4, 2]
+    for i in range(24):
+        assert oracle.unravel_index(i, shape) == list(np.unravel_index(i, shape))
+
+
+def test_slice_index_reference_literals():
+    # Test_Tensor.cpp:579-599.
+    data = np.arange(1, 7).astype(np.complex64)
+    cases = {("x", 0): (["y"], [1, 2, 3]), ("x", 1): (["y"], [4, 5, 6]),
+             ("y", 0): (["x"], [1, 4]), ("y", 1): (["x"], [2, 5]), ("y", 2): (["x"], [3, 6])}
+    for (label, value), (labels, vals) in cases.items():
+        lab, shp, out = oracle.slice_index(["x", "y"], [2, 3], data, label, value)
+        assert lab == labels and shp == [len(vals)]
+        assert np.array_equal(out, np.array(vals, np.complex64))
+
+
+def test_cpu_baseline_contract_matches_oracle():
+    if oracle.find_openblas() is None:
+        pytest.skip("no OpenBLAS with CBLAS symbols found")
+    oracle.load_blas(2)
+    rng = np.random.default_rng(5)
+    cases = [(["a", "b", "c", "d"], ["c", "e", "a"]),   # gemm
+             (["a", "b"], ["b", "a"]),                  # dot
+             (["a", "b", "c"], ["c"]),                  # gemv, right empty
+             (["c"], ["c", "b", "a"])]                  # gemv, left empty
+    for la, lb in cases:
+        sa, sb = [2] * len(la), [2] * len(lb)
+        a = (rng.standard_normal(2 ** len(la)) + 1j * rng.standard_normal(2 ** len(la))
+             ).astype(np.complex64)
+        b = (rng.standard_normal(2 ** len(lb)) + 1j * rng.standard_normal(2 ** len(lb))
+             ).astype(np.complex64)
+        got_labels, got = oracle.baseline_contract(la, sa, a, lb, sb, b, 1)
+        want_labels, _, want = oracle.contract(la, sa, a, lb, sb, b)
+        assert got_labels == want_labels
+        np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
