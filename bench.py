@@ -43,6 +43,34 @@ sys.path.insert(0, ROOT)
 METRIC = "TFLOP/s & slices/s, 7x7 grid amp, split-BF16 MFMA vs plain FP32, err<=1e-5"
 PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
 
+# Kernel names in rocprofv3 summaries for the event profiler's kernel families.
+ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": "cgemm_bf16x4m_kernel",
+                "gemm_bf16x3_mfma": "cgemm_bf16x3_kernel",
+                "gemm_fp32_mfma": "cgemm_embed_kernel f32",
+                "gemm_fp64_mfma": "cgemm_embed_kernel f64",
+                "permute_bits": "permute_bits_kernel c64"}
+
+
+def profiled_traffic(summary: str, family: str):
+    """HBM bytes per dispatch of a kernel, from the traffic table of a
+    committed rocprofv3 summary under profiles/ (FETCH_SIZE and WRITE_SIZE
+    passes, scaled by the copy-kernel calibration); None if absent."""
+    path = os.path.join(ROOT, "profiles", summary)
+    name = ROCPROF_NAME.get(family)
+    if name is None or not os.path.exists(path):
+        return None
+    in_traffic = False
+    for line in open(path):
+        if line.startswith("## HBM traffic"):
+            in_traffic = True
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if in_traffic and len(cells) >= 5 and cells[0] == name:
+            try:
+                return float(cells[4]) * 1e9
+            except ValueError:
+                return None
+    return None
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -241,7 +269,8 @@ def run_c1(args):
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK["bf16"], 4),
                      "hw_bf16_frac": round(achieved * (24 if args.split == "4m" else 18) / 8 /
                                            PEAK["bf16"], 4),
-                     "traffic": None},
+                     "traffic": profiled_traffic("c1_bf16x3_contraction.md", kname),
+                     "traffic_source": "profiles/c1_bf16x3_contraction.md"},
     }
     return result, (a_host, b_host, la, lb)
 
@@ -406,7 +435,9 @@ def run_c2(args):
                          for k, v in prof32.items()},
         "roofline": {"kernel": dname, "bound": bound, "achieved": round(achieved / scale, 3),
                      "peak": peak / scale, "unit": unit, "frac": round(achieved / peak, 4),
-                     "traffic": None},
+                     "algorithmic_bytes": round(dk["bytes"] / dk["launches"]),
+                     "traffic": profiled_traffic("c2_sliced_amplitude.md", dname),
+                     "traffic_source": "profiles/c2_sliced_amplitude.md"},
     }, (plan, net)
 
 

@@ -9,6 +9,11 @@ from ._native import JetHipError, Context, default_context
 from .tensor import (Tensor, contract_tensors, add_tensors, slice_index, transpose, reshape,
                      conj)
 from .permuter import Permuter
-from . import utilities
+from .path_info import PathInfo, PathStepInfo
+from .tensor_network import TensorNetwork
+from .task_based_contractor import TaskBasedContractor
+from .factory import (TensorC64, TensorC128, TensorNetworkC64, TensorNetworkC128,
+                      TaskBasedContractorC64, TaskBasedContractorC128)
+from . import factory, utilities
 
 __version__ = "0.1.0"

@@ -839,7 +839,7 @@ void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
     JH_HIP(hipGetLastError());
 }
 
-void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b,
+void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                          void *C, int64_t M, int64_t N, int64_t ldc,
                          int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                          hipStream_t stream)
@@ -851,7 +851,8 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b,
     JH_REQUIRE(batch <= 65535, "Batch count exceeds 65535.");
     JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
     dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);
-    const double K = double(a.k_p);
+    // Algorithmic work counts the unpadded K.
+    const double K = double(K_true);
     prof::Scope ps(prof::kGemmBf16x4m, stream, 8.0 * double(batch) * double(M) * double(N) * K,
                    8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
     hipLaunchKernelGGL((cgemm_bf16x4m_kernel<BM, BN, WM, WN>), grid, dim3(64 * WM * WN), 0,
@@ -860,7 +861,7 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b,
     JH_HIP(hipGetLastError());
 }
 
-void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b,
+void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                         void *C, int64_t M, int64_t N, int64_t ldc,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                         hipStream_t stream)
@@ -872,7 +873,7 @@ void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b,
     JH_REQUIRE(batch <= 65535, "Batch count exceeds 65535.");
     JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
     dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);
-    const double K = double(a.k_p);
+    const double K = double(K_true);
     prof::Scope ps(prof::kGemmBf16x3, stream, 8.0 * double(batch) * double(M) * double(N) * K,
                    8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
     hipLaunchKernelGGL((cgemm_bf16x3_kernel<BM, BN, WM, WN>), grid, dim3(64 * WM * WN), 0, stream,

@@ -86,11 +86,11 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
 void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
                        int64_t s, int64_t batch, const SplitOperand &dst,
                        hipStream_t stream);
-void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b,
+void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                          void *C, int64_t M, int64_t N, int64_t ldc,
                          int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                          hipStream_t stream);
-void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b,
+void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                         void *C, int64_t M, int64_t N, int64_t ldc,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                         hipStream_t stream);

@@ -154,9 +154,9 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         launch_split_bf16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
         launch_split_bf16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
         if (planes == 6)
-            launch_gemm_bf16x3(sa, sb, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
+            launch_gemm_bf16x3(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
         else
-            launch_gemm_bf16x4m(sa, sb, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
+            launch_gemm_bf16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
     }
     else {
         GemmArgs g;
