@@ -41,14 +41,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "TFLOP/s & slices/s, 7x7 grid amp, split-BF16 MFMA vs plain FP32, err<=1e-5"
-PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
+PEAK = {"bf16": 2.5e15, "fp16": 2.5e15, "fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
+
+
+def mfma_peak(family: str) -> float:
+    """Dense MFMA peak of the arithmetic a kernel family runs on."""
+    if "bf16" in family:
+        return PEAK["bf16"]
+    if "f16" in family:
+        return PEAK["fp16"]
+    if "fp64" in family:
+        return PEAK["fp64"]
+    return PEAK["fp32"]
+
+# Split 16-bit MFMA modes: precision constant name, network precision string,
+# event-profiler kernel family, 16-bit MFMA flops per algorithmic flop.
+SPLIT_MODES = {"3m": ("PREC_BF16X3", "bf16x3", "gemm_bf16x3_mfma", 18 / 8),
+               "4m": ("PREC_BF16X3_4M", "bf16x3_4m", "gemm_bf16x3_4m_mfma", 24 / 8),
+               "f16": ("PREC_F16X3_4M", "f16x3", "gemm_f16x3_4m_mfma", 24 / 8)}
+SPLIT_LABEL = {"3m": "split-BF16 3M", "4m": "split-BF16 4M", "f16": "split-FP16 4M"}
 
 # Kernel names in rocprofv3 summaries for the event profiler's kernel families.
-ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": "cgemm_bf16x4m_kernel",
-                "gemm_bf16x3_mfma": "cgemm_bf16x3_kernel",
-                "gemm_fp32_mfma": "cgemm_embed_kernel f32",
-                "gemm_fp64_mfma": "cgemm_embed_kernel f64",
-                "permute_bits": "permute_bits_kernel c64"}
+ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_bf16x4m_kernel"),
+                "gemm_f16x3_4m_mfma": ("cgemm_x4m_kernel f16",),
+                "gemm_bf16x3_mfma": ("cgemm_bf16x3_kernel",),
+                "gemm_fp32_mfma": ("cgemm_embed_kernel f32",),
+                "gemm_fp64_mfma": ("cgemm_embed_kernel f64",),
+                "permute_bits": ("permute_bits_kernel c64",)}
 
 
 def profiled_traffic(summary: str, family: str):
@@ -56,15 +75,15 @@ def profiled_traffic(summary: str, family: str):
     committed rocprofv3 summary under profiles/ (FETCH_SIZE and WRITE_SIZE
     passes, scaled by the copy-kernel calibration); None if absent."""
     path = os.path.join(ROOT, "profiles", summary)
-    name = ROCPROF_NAME.get(family)
-    if name is None or not os.path.exists(path):
+    names = ROCPROF_NAME.get(family)
+    if names is None or not os.path.exists(path):
         return None
     in_traffic = False
     for line in open(path):
         if line.startswith("## HBM traffic"):
             in_traffic = True
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if in_traffic and len(cells) >= 5 and cells[0] == name:
+        if in_traffic and len(cells) >= 5 and cells[0] in names:
             try:
                 return float(cells[4]) * 1e9
             except ValueError:
@@ -179,12 +198,12 @@ def run_c1(args):
             nat.Profiler.enable(False)
         return ms, wall, prof
 
-    split_prec = nat.PREC_BF16X3_4M if args.split == "4m" else nat.PREC_BF16X3
-    kname = "gemm_bf16x3_4m_mfma" if args.split == "4m" else "gemm_bf16x3_mfma"
-    log(f"c1: split-BF16 plan ({args.split})")
+    split_prec = getattr(nat, SPLIT_MODES[args.split][0])
+    kname = SPLIT_MODES[args.split][2]
+    log(f"c1: {SPLIT_LABEL[args.split]} plan")
     p16, ws16, wsz16 = plan(nat.C64, split_prec)
     ms16, wall16, prof16 = timed(p16, A, B, C, ws16, wsz16, args.steps, args.warmup, True)
-    log(f"c1: bf16x3 {ms16:.2f} ms/step {flops / ms16 / 1e9:.1f} TFLOP/s")
+    log(f"c1: {SPLIT_LABEL[args.split]} {ms16:.2f} ms/step {flops / ms16 / 1e9:.1f} TFLOP/s")
     c16 = np.empty(n, np.complex64)
     C.download(c16)
     del ws16
@@ -197,7 +216,7 @@ def run_c1(args):
             L.jet_hip_plan_destroy(p)
         return {"metric": METRIC, "value": round(flops / (ms16 * 1e-3) / 1e12, 3),
                 "unit": "TFLOP/s", "ms_per_step": round(ms16, 3),
-                "kernels_bf16x3": prof16}, None
+                "kernels_split": prof16}, None
 
     log("c1: FP32 plan")
     p32, ws32, wsz32 = plan(nat.C64, nat.PREC_FP32)
@@ -234,7 +253,8 @@ def run_c1(args):
     nref = np.linalg.norm(c64ref)
     err16 = float(np.linalg.norm(c16.astype(np.complex128) - c64ref) / nref)
     err32 = float(np.linalg.norm(c32.astype(np.complex128) - c64ref) / nref)
-    log(f"c1: err bf16x3 {err16:.3e} fp32 {err32:.3e} fp64-vs-cpu-sampled {fp64_vs_cpu:.2e}")
+    log(f"c1: err {SPLIT_LABEL[args.split]} {err16:.3e} fp32 {err32:.3e} "
+        f"fp64-vs-cpu-sampled {fp64_vs_cpu:.2e}")
     for p in (p16, p32, p64):
         L.jet_hip_plan_destroy(p)
 
@@ -247,7 +267,7 @@ def run_c1(args):
         "value": round(flops / (ms16 * 1e-3) / 1e12, 3),
         "unit": "TFLOP/s",
         "ms_per_step": round(ms16, 3),
-        "dtype": f"c64 (split-bf16x3 {args.split.upper()} MFMA, fp32 accumulate)",
+        "dtype": f"c64 ({SPLIT_LABEL[args.split]} MFMA, three terms, fp32 accumulate)",
         "config": {"workload": "c1: rank-28 x rank-28 complex64 contraction, 14 shared "
                                "indices at random positions, 2^14 x 2^14 x 2^14 complex GEMM",
                    "m": 1 << 14, "n": 1 << 14, "k": 1 << 14},
@@ -255,10 +275,11 @@ def run_c1(args):
                       "tflops": round(flops / (ms32 * 1e-3) / 1e12, 3)},
         "fp64_mfma": {"ms_per_step": round(ms64, 3),
                       "tflops": round(flops / (ms64 * 1e-3) / 1e12, 3)},
-        "speedup_bf16x3_vs_fp32": round(ms32 / ms16, 3),
-        "rel_err_vs_fp64": {"bf16x3": err16, "fp32": err32,
+        "split_mode": SPLIT_LABEL[args.split],
+        "speedup_split_vs_fp32": round(ms32 / ms16, 3),
+        "rel_err_vs_fp64": {"split": err16, "fp32": err32,
                             "fp64_vs_cpu_exact_sampled": fp64_vs_cpu},
-        "kernels_bf16x3": {k: {"launches": v["launches"],
+        "kernels_split": {k: {"launches": v["launches"],
                                "ms_avg": round(v["ms"] / v["launches"], 4)}
                            for k, v in prof16.items()},
         "kernels_fp32": {k: {"launches": v["launches"],
@@ -267,10 +288,10 @@ def run_c1(args):
         "roofline": {"kernel": kname, "bound": "mfma",
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK["bf16"] / 1e12,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK["bf16"], 4),
-                     "hw_bf16_frac": round(achieved * (24 if args.split == "4m" else 18) / 8 /
-                                           PEAK["bf16"], 4),
-                     "traffic": profiled_traffic("c1_bf16x3_contraction.md", kname),
-                     "traffic_source": "profiles/c1_bf16x3_contraction.md"},
+                     "hw_16bit_mfma_frac": round(achieved * SPLIT_MODES[args.split][3] /
+                                                 PEAK["bf16"], 4),
+                     "traffic": profiled_traffic("c1_split_contraction.md", kname),
+                     "traffic_source": "profiles/c1_split_contraction.md"},
     }
     return result, (a_host, b_host, la, lb)
 
@@ -356,17 +377,29 @@ def run_c2(args):
         return ms, prof
 
     steps, warm = args.steps, args.warmup
-    split = "bf16x3_4m" if args.split == "4m" else "bf16x3"
-    sc16 = build(split, np.complex64)
+    label = SPLIT_LABEL[args.split]
+    sc16 = build(SPLIT_MODES[args.split][1], np.complex64)
     n_batch = sc16.n_batch
     exec_flops = sc16.flops_per_execution
-    log(f"c2: bf16x3 network: {exec_flops:.3e} flops/execution, arena "
+    log(f"c2: {label} network: {exec_flops:.3e} flops/execution, arena "
         f"{sc16.net.arena_bytes / 2**30:.1f} GiB, workspace {sc16.net.workspace_bytes / 2**30:.1f} GiB")
     ms16, prof16 = timed(sc16, steps, warm, True)
     res16 = sc16.slice_results((warm + steps) * n_batch)
-    sum16 = sc16.reduce((warm + steps) * n_batch)
-    log(f"c2: bf16x3 {ms16 / steps:.1f} ms/execution, {exec_flops * steps / ms16 / 1e9:.1f} TFLOP/s")
+    log(f"c2: {label} {ms16 / steps:.1f} ms/execution, "
+        f"{exec_flops * steps / ms16 / 1e9:.1f} TFLOP/s")
     del sc16
+
+    # The other split modes, timed the same way for comparison.
+    others = {}
+    for mode in args.compare:
+        if mode == args.split:
+            continue
+        sc = build(SPLIT_MODES[mode][1], np.complex64)
+        ms, _ = timed(sc, steps, warm, False)
+        others[mode] = (ms, sc.slice_results(n_batch))
+        log(f"c2: {SPLIT_LABEL[mode]} {ms / steps:.1f} ms/execution, "
+            f"{exec_flops * steps / ms / 1e9:.1f} TFLOP/s")
+        del sc
 
     steps32 = max(1, args.steps32)
     sc32 = build("fp32", np.complex64)
@@ -387,8 +420,18 @@ def run_c2(args):
     err32 = float(np.linalg.norm(res32[:n_ref] - res64) / np.linalg.norm(res64))
     amp16 = complex(np.sum(res16[:n_ref].astype(np.complex128)))
     amp_err16 = abs(amp16 - sum64) / abs(sum64)
-    log(f"c2: fp64 {ms64:.1f} ms/execution; rel err per-slice bf16x3 {err16:.2e} fp32 {err32:.2e}; "
-        f"partial-amplitude err {amp_err16:.2e}")
+    log(f"c2: fp64 {ms64:.1f} ms/execution; rel err per-slice {label} {err16:.2e} "
+        f"fp32 {err32:.2e}; partial-amplitude err {amp_err16:.2e}")
+    compare = {}
+    for mode, (ms, res) in others.items():
+        e = float(np.linalg.norm(res[:n_ref] - res64) / np.linalg.norm(res64))
+        compare[SPLIT_LABEL[mode]] = {
+            "ms_per_step": round(ms / steps, 3),
+            "tflops": round(exec_flops * steps / (ms * 1e-3) / 1e12, 3),
+            "slices_per_s": round(steps * n_batch / (ms * 1e-3), 4),
+            "speedup_vs_fp32": round((ms32 / steps32) / (ms / steps), 3),
+            "rel_err_per_slice_vs_fp64": e}
+        log(f"c2: {SPLIT_LABEL[mode]} rel err per-slice {e:.2e}")
 
     slices16 = steps * n_batch
     tflops16 = exec_flops * steps / (ms16 * 1e-3)
@@ -400,7 +443,7 @@ def run_c2(args):
     bound = "mfma" if dk["flops"] > 0 else "hbm"
     if bound == "mfma":
         achieved = dk["flops"] / (dk["ms"] * 1e-3)
-        peak = PEAK["bf16"] if "bf16" in dname else PEAK["fp32"]
+        peak = mfma_peak(dname)
         unit, scale = "TFLOP/s", 1e12
     else:
         achieved = dk["bytes"] / (dk["ms"] * 1e-3)
@@ -410,7 +453,8 @@ def run_c2(args):
         "value": round(tflops16 / 1e12, 3),
         "unit": "TFLOP/s",
         "ms_per_step": round(ms16 / steps, 3),
-        "dtype": "c64 (split-bf16x3 MFMA on MFMA-bound steps, fp32 elsewhere)",
+        "dtype": f"c64 ({label} MFMA, three terms, fp32 accumulate, on MFMA-bound steps; "
+                 "fp32 MFMA elsewhere)",
         "config": {"workload": "c2: one amplitude of a seeded 7x7-qubit grid random circuit, "
                                "depth 14, complex64, min-fill path sliced on 12 bond-2 indices "
                                "(4096 slices); a step is one batched execution of "
@@ -423,18 +467,22 @@ def run_c2(args):
         "slices_per_s": round(sps16, 4),
         "fp32_mfma": {"ms_per_step": round(ms32 / steps32, 3), "tflops": round(tflops32 / 1e12, 3),
                       "slices_per_s": round(sps32, 4)},
-        "speedup_bf16x3_vs_fp32": round(tflops16 / tflops32, 3),
-        "projected_amplitude_s": {"bf16x3": round(4096 / sps16, 1), "fp32": round(4096 / sps32, 1)},
-        "rel_err_vs_fp64": {"bf16x3_per_slice": err16, "fp32_per_slice": err32,
-                            "bf16x3_partial_amplitude": amp_err16,
+        "split_mode": label,
+        "speedup_split_vs_fp32": round(tflops16 / tflops32, 3),
+        "projected_amplitude_s": {"split": round(4096 / sps16, 1), "fp32": round(4096 / sps32, 1)},
+        "rel_err_vs_fp64": {"split_per_slice": err16, "fp32_per_slice": err32,
+                            "split_partial_amplitude": amp_err16,
                             "slices_compared": n_ref},
+        "other_split_modes": compare,
         "fp64_mfma": {"ms_per_step": round(ms64, 3)},
-        "kernels_bf16x3": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
-                           for k, v in prof16.items()},
+        "kernels_split": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
+                          for k, v in prof16.items()},
         "kernels_fp32": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
                          for k, v in prof32.items()},
         "roofline": {"kernel": dname, "bound": bound, "achieved": round(achieved / scale, 3),
                      "peak": peak / scale, "unit": unit, "frac": round(achieved / peak, 4),
+                     "hw_16bit_mfma_frac": (round(achieved * SPLIT_MODES[args.split][3] / peak, 4)
+                                            if dname == SPLIT_MODES[args.split][2] else None),
                      "algorithmic_bytes": round(dk["bytes"] / dk["launches"]),
                      "traffic": profiled_traffic("c2_sliced_amplitude.md", dname),
                      "traffic_source": "profiles/c2_sliced_amplitude.md"},
@@ -496,8 +544,10 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--batch-log2", type=int, default=2,
                     help="c2: sliced indices batched per execution (2^b slices per step)")
-    ap.add_argument("--split", default="4m", choices=["3m", "4m"],
+    ap.add_argument("--split", default="4m", choices=sorted(SPLIT_MODES),
                     help="complex form of the split-BF16 GEMM")
+    ap.add_argument("--compare", nargs="*", default=["4m", "f16"], choices=sorted(SPLIT_MODES),
+                    help="c2: other split modes timed for comparison")
     ap.add_argument("--steps32", type=int, default=1,
                     help="c2: executions timed in plain FP32")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -65,10 +65,16 @@ typedef enum {
                                  product, FP32 accumulation.  C64 only.  Steps
                                  the plan finds HBM-bound run on FP32 MFMA.   */
     JET_HIP_PREC_FP64 = 3,    /* 4M complex GEMM on FP64 MFMA (C128 only)      */
-    JET_HIP_PREC_BF16X3_4M = 4 /* as BF16X3, but the complex product in 4M form
-                                  (12 BF16 MFMA per real-product triple; twice
-                                  the MFMA work per operand byte of 3M and no
-                                  Im = T3 - T1 - T2 cancellation)              */
+    JET_HIP_PREC_BF16X3_4M = 4, /* as BF16X3, but the complex product in 4M
+                                  form (12 BF16 MFMA per real-product triple;
+                                  twice the MFMA work per operand byte of 3M
+                                  and no Im = T3 - T1 - T2 cancellation)      */
+    JET_HIP_PREC_F16X3_4M = 5  /* as BF16X3_4M with FP16 pieces: each operand
+                                  row is scaled by a power of two into the
+                                  FP16 range, split into hi+lo FP16 (22
+                                  significant bits, BF16 keeps 16) and the
+                                  scales are undone on the outputs.  Same
+                                  MFMA rate; error close to FP32 MFMA.  C64.  */
 } jet_hip_precision_t;
 
 typedef struct jet_hip_ctx_s *jet_hip_ctx_t;

@@ -24,9 +24,10 @@ PREC_FP32 = 1
 PREC_BF16X3 = 2
 PREC_FP64 = 3
 PREC_BF16X3_4M = 4
+PREC_F16X3_4M = 5
 
 GEMM_KERNEL_NAMES = {0: "valu", 1: "valu_reduce", 2: "fp32_mfma", 3: "fp64_mfma", 4: "bf16x3_mfma",
-                     5: "bf16x3_4m_mfma", 6: "splitk"}
+                     5: "bf16x3_4m_mfma", 6: "splitk", 7: "f16x3_4m_mfma"}
 
 # Exported symbols (must match include/jet_hip.h).
 EXPORTS = [

@@ -40,6 +40,7 @@ int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
         // (every dimension large enough to reuse operands from LDS).
         if (mode == 1 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x3;
         if (mode == 3 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x4m;
+        if (mode == 4 && M >= 128 && N >= 128 && K >= 128) return kGemmF16x4m;
         return kGemmF32Mfma;
     }
     if (K >= 4 * kSplitKChunk && batch * M * N <= 65536) return kGemmSplitK;
@@ -57,7 +58,9 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
     JH_REQUIRE(ra >= 0 && rb >= 0 && rc >= 0 && ra <= kMaxRank && rb <= kMaxRank &&
                    rc <= kMaxRank,
                "Tensor rank must be between 0 and 64.");
+    JH_REQUIRE(mode >= 0 && mode <= 4, "Unknown precision mode.");
     JH_REQUIRE(!((mode == 1 || mode == 3) && elem_bytes != 8), "Split-BF16 needs complex64 data.");
+    JH_REQUIRE(!(mode == 4 && elem_bytes != 8), "Split-FP16 needs complex64 data.");
     JH_REQUIRE(!(mode == 2 && elem_bytes != 16), "FP64 MFMA needs complex128 data.");
     JH_REQUIRE(!(mode == 0 && elem_bytes != 8), "FP32 MFMA needs complex64 data.");
 
@@ -176,7 +179,8 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
         d.off_ct = off;
         off = align256(off + eb * (size_t)d.size_c);
     }
-    if (d.gemm_kernel == kGemmBf16x3 || d.gemm_kernel == kGemmBf16x4m) {
+    if (d.gemm_kernel == kGemmBf16x3 || d.gemm_kernel == kGemmBf16x4m ||
+        d.gemm_kernel == kGemmF16x4m) {
         const size_t planes = d.gemm_kernel == kGemmBf16x3 ? 6 : 4;
         d.split_a_plane = split_plane_elems(d.batch, d.M, d.K);
         d.split_b_plane = split_plane_elems(d.batch, d.N, d.K);
@@ -184,6 +188,18 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
         off = align256(off + planes * 2 * (size_t)d.split_a_plane);
         d.off_sb = off;
         off = align256(off + planes * 2 * (size_t)d.split_b_plane);
+        if (d.gemm_kernel == kGemmF16x4m) {
+            const size_t ra = (size_t)d.batch * (size_t)(ceil_div(d.M, kSplitRowPad) * kSplitRowPad);
+            const size_t rb = (size_t)d.batch * (size_t)(ceil_div(d.N, kSplitRowPad) * kSplitRowPad);
+            d.off_rmax_a = off;
+            off = align256(off + 4 * ra);
+            d.off_scale_a = off;
+            off = align256(off + 4 * ra);
+            d.off_rmax_b = off;
+            off = align256(off + 4 * rb);
+            d.off_scale_b = off;
+            off = align256(off + 4 * rb);
+        }
     }
     if (d.gemm_kernel == kGemmSplitK) {
         d.off_partials = off;

@@ -26,7 +26,7 @@ namespace jet_hip {
 
 struct ContractionDesc {
     int elem_bytes = 8; // 8 complex64, 16 complex128
-    int mode = 0;       // 0 fp32, 1 bf16x3, 2 fp64
+    int mode = 0;       // 0 fp32, 1 bf16x3 3M, 2 fp64, 3 bf16x3 4M, 4 f16x3 4M
     int64_t batch = 1, M = 1, N = 1, K = 1;
     int64_t size_a = 1, size_b = 1, size_c = 1;
     bool need_a = false, need_b = false, need_c = false;
@@ -34,6 +34,8 @@ struct ContractionDesc {
     int gemm_kernel = kGemmSmall;
     // Workspace layout (bytes, 256-aligned offsets).
     size_t off_at = 0, off_bt = 0, off_ct = 0, off_sa = 0, off_sb = 0, off_partials = 0;
+    // split-FP16 row maxima and scales of A and B (batch * padded rows each)
+    size_t off_rmax_a = 0, off_scale_a = 0, off_rmax_b = 0, off_scale_b = 0;
     int64_t split_a_plane = 0, split_b_plane = 0;
     size_t workspace = 0;
     double flops = 0; // 8 * batch * M * N * K

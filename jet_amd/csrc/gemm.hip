@@ -32,6 +32,7 @@ namespace {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 template <class R> struct Vec;
 // Output row of accumulator item r held by lane group q (= lane / 16) in a
@@ -334,6 +335,112 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Split-FP16 operand preparation.  fp16 keeps 11 significant bits, so
+// hi + lo carries 22 (bf16: 16), but its range is narrow: each row is first
+// scaled by a power of two that brings its largest |re| or |im| into
+// [2^14, 2^15), the scale being undone on the GEMM outputs.
+//   1. row_max_clear_kernel: rowmax[] = 0;
+//   2. row_absmax_kernel: rowmax[row] = max over the row of the FP32 bit
+//      patterns of |re| and |im| (integer order = magnitude order for
+//      non-negative floats), one work-group per (row, 2048-element chunk);
+//   3. split_f16_kernel: scaled planes and scale[row] = 2^(e - 14).
+// ---------------------------------------------------------------------------
+constexpr int kRowMaxChunk = 2048;
+
+__global__ __launch_bounds__(256) void row_max_clear_kernel(uint32_t *rowmax, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        rowmax[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void row_absmax_kernel(
+    const float2 *__restrict__ src, int64_t rows, int64_t K, int64_t ld, int64_t s,
+    int64_t batch, int64_t rows_p, uint32_t *__restrict__ rowmax)
+{
+    __shared__ uint32_t red[4];
+    const int64_t nch = (K + kRowMaxChunk - 1) / kRowMaxChunk;
+    const int64_t units = batch * rows * nch;
+    for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const int64_t ch = u % nch;
+        const int64_t rest = u / nch;
+        const int64_t r = rest % rows;
+        const int64_t b = rest / rows;
+        const float2 *row = src + b * s + r * ld;
+        const int64_t k1 = (ch + 1) * kRowMaxChunk < K ? (ch + 1) * kRowMaxChunk : K;
+        uint32_t m = 0u;
+        for (int64_t k = ch * kRowMaxChunk + threadIdx.x; k < k1; k += 256) {
+            const float2 x = row[k];
+            const uint32_t xr = __float_as_uint(x.x) & 0x7fffffffu;
+            const uint32_t xi = __float_as_uint(x.y) & 0x7fffffffu;
+            m = max(m, max(xr, xi));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            atomicMax(rowmax + b * rows_p + r, max(max(red[0], red[1]), max(red[2], red[3])));
+        __syncthreads();
+    }
+}
+
+// Power-of-two exponent that scales a row whose largest magnitude has FP32
+// bit pattern m into [2^14, 2^15); clamped so that 2^sh stays a normal float.
+__device__ __forceinline__ int row_shift(uint32_t m)
+{
+    if (m == 0u) return 0;
+    const int e = int(m >> 23) - 127; // floor(log2) for normal values
+    const int sh = 14 - e;
+    return sh > 126 ? 126 : (sh < -126 ? -126 : sh);
+}
+
+__device__ __forceinline__ void split2h(float v, _Float16 &h, _Float16 &l)
+{
+    h = (_Float16)v;
+    l = (_Float16)(v - (float)h);
+}
+
+__global__ __launch_bounds__(256) void split_f16_kernel(
+    const float2 *__restrict__ src, int64_t rows, int64_t K, int64_t ld,
+    int64_t s, SplitOperand dst)
+{
+    const int64_t kq_n = dst.k_p / 8;
+    const int64_t total = dst.batch * dst.rows_p * kq_n;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    _Float16 *base = static_cast<_Float16 *>(dst.base);
+    for (int64_t gi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; gi < total;
+         gi += stride) {
+        const int64_t kq = gi % kq_n;
+        const int64_t rest = gi / kq_n;
+        const int64_t r = rest % dst.rows_p;
+        const int64_t b = rest / dst.rows_p;
+        const int64_t row = b * dst.rows_p + r;
+        const int sh = r < rows ? row_shift(dst.rowmax[row]) : 0;
+        const float sc = __builtin_ldexpf(1.f, sh);
+        if (kq == 0) dst.scale[row] = __builtin_ldexpf(1.f, -sh);
+        f16x8 o[4];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int64_t k = kq * 8 + e;
+            float2 x = {0.f, 0.f};
+            if (r < rows && k < K) x = src[b * s + r * ld + k];
+            _Float16 h, l;
+            split2h(x.x * sc, h, l);
+            o[0][e] = h;
+            o[1][e] = l;
+            split2h(x.y * sc, h, l);
+            o[2][e] = h;
+            o[3][e] = l;
+        }
+        // [row][k tile][plane][32], as the split-BF16 4M layout
+        const int64_t off = (row * (dst.k_p / 32) + kq / 4) * 128 + (kq % 4) * 8;
+#pragma unroll
+        for (int p = 0; p < 4; p++) *reinterpret_cast<f16x8 *>(base + off + p * 32) = o[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // 3M split-BF16 complex GEMM.
 // Stage s covers product s/nk (T1, T2, T3) and k tile s%nk: it loads the hi
 // and lo planes of that product's operands (BM x 32 and BN x 32 bf16 each)
@@ -500,19 +607,42 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x3_kernel(
 // many MFMAs as in the 3M form, and no T3 - T1 - T2 cancellation occurs.
 // Double-buffered LDS with the next k tile prefetched into registers.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bf16x8 neg_bf16x8(bf16x8 v)
+// Sign flip of eight 16-bit floats (bf16 and fp16 keep the sign in bit 15).
+template <class V> __device__ __forceinline__ V neg16x8(V v)
 {
     typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
     u32x4 u = __builtin_bit_cast(u32x4, v);
     u ^= 0x80008000u;
-    return __builtin_bit_cast(bf16x8, u);
+    return __builtin_bit_cast(V, u);
 }
 
-template <int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
+// The 16x16x32 MFMA of each 16-bit operand type: lane l supplies 8
+// consecutive k (8(l/16)..8(l/16)+7) of row l%16 of A and of column l%16
+// of B, and holds D[4(l/16)+r][l%16].
+template <class E> struct Mfma16;
+template <> struct Mfma16<__bf16> {
+    using vec = bf16x8;
+    static __device__ __forceinline__ f32x4 mma(vec a, vec b, f32x4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma16<_Float16> {
+    using vec = f16x8;
+    static __device__ __forceinline__ f32x4 mma(vec a, vec b, f32x4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// E = __bf16: split-BF16; E = _Float16: split-FP16, whose operand rows carry
+// power-of-two scales that the epilogue applies (a.scale / b.scale).
+template <class E, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_kernel(
     SplitOperand a, SplitOperand b, float2 *__restrict__ Cout, int64_t M,
     int64_t N, int64_t ldc, int64_t sC, int a_bcast, int b_bcast)
 {
+    using vec = typename Mfma16<E>::vec;
     constexpr int NT = 64 * WM * WN;
     constexpr int KT = 32;
     constexpr int WTM = BM / WM, WTN = BN / WN;
@@ -520,11 +650,11 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
     constexpr int A_CH = 4 * BM * (KT / 8) / NT; // 16-byte chunks per thread
     constexpr int B_CH = 4 * BN * (KT / 8) / NT;
     static_assert(A_CH * NT == 4 * BM * 4 && B_CH * NT == 4 * BN * 4, "tile");
-    // LDS rows of 32 bf16 (four 16-byte chunks) without padding; chunk kc of
+    // LDS rows of 32 16-bit values (four 16-byte chunks) without padding; chunk kc of
     // row r is stored at kc ^ ((r >> 2) & 3), which spreads the 16 rows read
     // by one MFMA operand fetch over all 64 banks.
-    __shared__ bf16x8 As[2][4 * BM * 4];
-    __shared__ bf16x8 Bs[2][4 * BN * 4];
+    __shared__ vec As[2][4 * BM * 4];
+    __shared__ vec Bs[2][4 * BN * 4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
@@ -532,10 +662,10 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
     int64_t tm, tn;
     tile_coords(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
     const int64_t m0 = tm * BM, n0 = tn * BN;
-    // interleaved layout: a batch holds rows_p * k_p * 4 bf16
-    const __bf16 *Ab = static_cast<const __bf16 *>(a.base) +
+    // interleaved layout: a batch holds rows_p * k_p * 4 values
+    const E *Ab = static_cast<const E *>(a.base) +
                        (a_bcast ? 0 : bz * a.rows_p * a.k_p * 4);
-    const __bf16 *Bb = static_cast<const __bf16 *>(b.base) +
+    const E *Bb = static_cast<const E *>(b.base) +
                        (b_bcast ? 0 : bz * b.rows_p * b.k_p * 4);
 
     f32x4 cr[FM][FN], ci[FM][FN];
@@ -547,8 +677,8 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
             ci[i][jj] = f32x4{0, 0, 0, 0};
         }
 
-    bf16x8 ra[A_CH], rb[B_CH];
-    const bf16x8 zero = {};
+    vec ra[A_CH], rb[B_CH];
+    const vec zero = {};
     // Chunk c of a tile: row c / 16, then 16 chunks of 16 bytes covering the
     // 256 contiguous bytes of that row's k tile (plane (c % 16) / 4, column
     // chunk c % 4).
@@ -559,7 +689,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
             const int c = tid + i * NT;
             const int r = c / 16, w = c % 16;
             const int64_t m = m0 + r;
-            ra[i] = (m < a.rows_p) ? *reinterpret_cast<const bf16x8 *>(
+            ra[i] = (m < a.rows_p) ? *reinterpret_cast<const vec *>(
                                          Ab + ((m * nkt_a + kt) * 16 + w) * 8)
                                    : zero;
         }
@@ -568,7 +698,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
             const int c = tid + i * NT;
             const int r = c / 16, w = c % 16;
             const int64_t n = n0 + r;
-            rb[i] = (n < b.rows_p) ? *reinterpret_cast<const bf16x8 *>(
+            rb[i] = (n < b.rows_p) ? *reinterpret_cast<const vec *>(
                                          Bb + ((n * nkt_b + kt) * 16 + w) * 8)
                                    : zero;
         }
@@ -596,7 +726,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
     for (int kt = 0; kt < nk; kt++) {
         const int buf = kt & 1;
         if (kt + 1 < nk) gload(kt + 1);
-        bf16x8 brh[FN], brl[FN], bih[FN], bil[FN];
+        vec brh[FN], brl[FN], bih[FN], bil[FN];
 #pragma unroll
         for (int fn = 0; fn < FN; fn++) {
             const int r = wn * WTN + fn * 16 + j;
@@ -610,26 +740,26 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
         for (int fm = 0; fm < FM; fm++) {
             const int r = wm * WTM + fm * 16 + j;
             const int sw = q ^ ((r >> 2) & 3);
-            const bf16x8 arh = As[buf][(0 * BM + r) * 4 + sw];
-            const bf16x8 arl = As[buf][(1 * BM + r) * 4 + sw];
-            const bf16x8 aih = As[buf][(2 * BM + r) * 4 + sw];
-            const bf16x8 ail = As[buf][(3 * BM + r) * 4 + sw];
-            const bf16x8 naih = neg_bf16x8(aih), nail = neg_bf16x8(ail);
+            const vec arh = As[buf][(0 * BM + r) * 4 + sw];
+            const vec arl = As[buf][(1 * BM + r) * 4 + sw];
+            const vec aih = As[buf][(2 * BM + r) * 4 + sw];
+            const vec ail = As[buf][(3 * BM + r) * 4 + sw];
+            const vec naih = neg16x8(aih), nail = neg16x8(ail);
 #pragma unroll
             for (int fn = 0; fn < FN; fn++) {
                 f32x4 x = cr[fm][fn], y = ci[fm][fn];
-                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brh[fn], x, 0, 0, 0);
-                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bih[fn], y, 0, 0, 0);
-                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brl[fn], x, 0, 0, 0);
-                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bil[fn], y, 0, 0, 0);
-                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, brh[fn], x, 0, 0, 0);
-                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, bih[fn], y, 0, 0, 0);
-                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bih[fn], x, 0, 0, 0);
-                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh[fn], y, 0, 0, 0);
-                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bil[fn], x, 0, 0, 0);
-                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl[fn], y, 0, 0, 0);
-                x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, bih[fn], x, 0, 0, 0);
-                y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh[fn], y, 0, 0, 0);
+                x = Mfma16<E>::mma(arh, brh[fn], x);
+                y = Mfma16<E>::mma(arh, bih[fn], y);
+                x = Mfma16<E>::mma(arh, brl[fn], x);
+                y = Mfma16<E>::mma(arh, bil[fn], y);
+                x = Mfma16<E>::mma(arl, brh[fn], x);
+                y = Mfma16<E>::mma(arl, bih[fn], y);
+                x = Mfma16<E>::mma(naih, bih[fn], x);
+                y = Mfma16<E>::mma(aih, brh[fn], y);
+                x = Mfma16<E>::mma(naih, bil[fn], x);
+                y = Mfma16<E>::mma(aih, brl[fn], y);
+                x = Mfma16<E>::mma(nail, bih[fn], x);
+                y = Mfma16<E>::mma(ail, brh[fn], y);
                 cr[fm][fn] = x;
                 ci[fm][fn] = y;
             }
@@ -639,15 +769,23 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x4m_kernel(
     }
 
     float2 *C = Cout + bz * sC;
+    const float *sa = a.scale ? a.scale + (a_bcast ? 0 : bz) * a.rows_p : nullptr;
+    const float *sb = b.scale ? b.scale + (b_bcast ? 0 : bz) * b.rows_p : nullptr;
 #pragma unroll
     for (int fm = 0; fm < FM; fm++) {
 #pragma unroll
         for (int fn = 0; fn < FN; fn++) {
             const int64_t col = n0 + wn * WTN + fn * 16 + j;
+            const float fb = (sb && col < N) ? sb[col] : 1.f;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int64_t row = m0 + wm * WTM + fm * 16 + 4 * q + r;
-                if (row < M && col < N) C[row * ldc + col] = float2{cr[fm][fn][r], ci[fm][fn][r]};
+                if (row < M && col < N) {
+                    // Power-of-two factors, applied one at a time so that
+                    // their product cannot leave the float range early.
+                    const float fa = sa ? sa[row] : 1.f;
+                    C[row * ldc + col] = float2{cr[fm][fn][r] * fa * fb, ci[fm][fn][r] * fa * fb};
+                }
             }
         }
     }
@@ -839,6 +977,53 @@ void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
     JH_HIP(hipGetLastError());
 }
 
+void launch_split_f16(const void *src, int64_t rows, int64_t k, int64_t ld,
+                      int64_t s, int64_t batch, const SplitOperand &dst,
+                      hipStream_t stream)
+{
+    JH_REQUIRE(dst.planes == 4 && dst.interleaved && dst.rowmax && dst.scale,
+               "Split-FP16 operands need four interleaved planes and row scales.");
+    const int64_t nrow = dst.batch * dst.rows_p;
+    const int64_t total = nrow * (dst.k_p / 8);
+    const int64_t units = batch * rows * ceil_div(k, int64_t(kRowMaxChunk));
+    // reads the operand twice (row maxima, then the split) and writes 4 fp16
+    // per (padded) element
+    prof::Scope ps(prof::kSplitF16, stream, 0.0,
+                   16.0 * double(batch) * double(rows) * double(k) +
+                       8.0 * double(nrow) * double(dst.k_p));
+    hipLaunchKernelGGL(row_max_clear_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(nrow, 256), 4096)),
+                       dim3(256), 0, stream, dst.rowmax, nrow);
+    if (units > 0)
+        hipLaunchKernelGGL(row_absmax_kernel, dim3((unsigned)std::min<int64_t>(units, 65536)),
+                           dim3(256), 0, stream, static_cast<const float2 *>(src), rows, k, ld, s,
+                           batch, dst.rows_p, dst.rowmax);
+    hipLaunchKernelGGL(split_f16_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)),
+                       dim3(256), 0, stream, static_cast<const float2 *>(src), rows, k, ld, s, dst);
+    JH_HIP(hipGetLastError());
+}
+
+void launch_gemm_f16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
+                        void *C, int64_t M, int64_t N, int64_t ldc,
+                        int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
+                        hipStream_t stream)
+{
+    constexpr int BM = 128, BN = 128, WM = 2, WN = 2;
+    JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
+    JH_REQUIRE(a.planes == 4 && b.planes == 4 && a.interleaved && b.interleaved && a.scale &&
+                   b.scale,
+               "The split-FP16 kernel needs four interleaved planes and row scales.");
+    JH_REQUIRE(batch <= 65535, "Batch count exceeds 65535.");
+    JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
+    dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);
+    const double K = double(K_true);
+    prof::Scope ps(prof::kGemmF16x4m, stream, 8.0 * double(batch) * double(M) * double(N) * K,
+                   8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
+    hipLaunchKernelGGL((cgemm_x4m_kernel<_Float16, BM, BN, WM, WN>), grid, dim3(64 * WM * WN), 0,
+                       stream, a, b, static_cast<float2 *>(C), M, N, ldc, sC, a_bcast ? 1 : 0,
+                       b_bcast ? 1 : 0);
+    JH_HIP(hipGetLastError());
+}
+
 void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                          void *C, int64_t M, int64_t N, int64_t ldc,
                          int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
@@ -855,7 +1040,7 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K
     const double K = double(K_true);
     prof::Scope ps(prof::kGemmBf16x4m, stream, 8.0 * double(batch) * double(M) * double(N) * K,
                    8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
-    hipLaunchKernelGGL((cgemm_bf16x4m_kernel<BM, BN, WM, WN>), grid, dim3(64 * WM * WN), 0,
+    hipLaunchKernelGGL((cgemm_x4m_kernel<__bf16, BM, BN, WM, WN>), grid, dim3(64 * WM * WN), 0,
                        stream, a, b, static_cast<float2 *>(C), M, N, ldc, sC, a_bcast ? 1 : 0,
                        b_bcast ? 1 : 0);
     JH_HIP(hipGetLastError());

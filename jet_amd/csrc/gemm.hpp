@@ -33,6 +33,8 @@ enum GemmKernel {
     kGemmBf16x4m = 5,   // 4M complex, split-BF16, v_mfma_f32_16x16x32_bf16
     kGemmSplitK = 6,    // VALU, few outputs and long k: k split over blocks,
                         // partial sums combined in FP64
+    kGemmF16x4m = 7,    // 4M complex, split-FP16 with per-row power-of-two
+                        // scales, v_mfma_f32_16x16x32_f16
 };
 
 // Split-K: k range per block, and the FP64 partial-sum buffer it needs.
@@ -57,11 +59,18 @@ constexpr int kSplitKPad = 32;
 //  * interleaved (4M kernel, four planes): [batch][rows_p][k_p / 32][4][32]
 //    bf16, i.e. the four planes of one row and one 32-wide k tile form 256
 //    contiguous bytes (two full cache lines per row and k tile).
+//  * split-FP16 (4M kernel, four interleaved fp16 planes): each row r of
+//    batch b is scaled by 2^(14 - e) before the split, e the exponent of the
+//    row's largest |re| or |im|; rowmax[b * rows_p + r] holds that largest
+//    magnitude's FP32 bit pattern and scale[b * rows_p + r] = 2^(e - 14), the
+//    factor the GEMM applies to its outputs.
 struct SplitOperand {
     void *base;
     int64_t rows_p, k_p, batch, plane_elems;
     int planes;
     int interleaved;
+    uint32_t *rowmax = nullptr; // split-FP16 only
+    float *scale = nullptr;     // split-FP16 only
 };
 
 inline int64_t split_plane_elems(int64_t batch, int64_t rows, int64_t k)
@@ -73,7 +82,7 @@ inline int64_t split_plane_elems(int64_t batch, int64_t rows, int64_t k)
 
 // Which kernel the plan uses for a given shape and precision mode
 // (0 = fp32/complex64, 1 = split-bf16 3M/complex64, 2 = fp64/complex128,
-//  3 = split-bf16 4M/complex64).
+//  3 = split-bf16 4M/complex64, 4 = split-fp16 4M/complex64).
 int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
                        int64_t batch);
 
@@ -90,6 +99,14 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K
                          void *C, int64_t M, int64_t N, int64_t ldc,
                          int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                          hipStream_t stream);
+// Row maxima then the scaled fp16 planes of a split-FP16 operand.
+void launch_split_f16(const void *src, int64_t rows, int64_t k, int64_t ld,
+                      int64_t s, int64_t batch, const SplitOperand &dst,
+                      hipStream_t stream);
+void launch_gemm_f16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
+                        void *C, int64_t M, int64_t N, int64_t ldc,
+                        int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
+                        hipStream_t stream);
 void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                         void *C, int64_t M, int64_t N, int64_t ldc,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,

@@ -99,6 +99,8 @@ int mode_of(jet_hip_dtype_t dt, jet_hip_precision_t p)
         return 2;
     case JET_HIP_PREC_BF16X3_4M:
         return 3;
+    case JET_HIP_PREC_F16X3_4M:
+        return 4;
     }
     JH_FAIL(JET_HIP_ERROR_INVALID_VALUE, "Unknown precision.");
 }
@@ -143,7 +145,8 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         Bt = ws + d.off_bt;
     }
     void *Ct = d.need_c ? static_cast<void *>(ws + d.off_ct) : c;
-    if (d.gemm_kernel == kGemmBf16x3 || d.gemm_kernel == kGemmBf16x4m) {
+    if (d.gemm_kernel == kGemmBf16x3 || d.gemm_kernel == kGemmBf16x4m ||
+        d.gemm_kernel == kGemmF16x4m) {
         const int planes = d.gemm_kernel == kGemmBf16x3 ? 6 : 4;
         SplitOperand sa{ws + d.off_sa, ceil_div(d.M, kSplitRowPad) * kSplitRowPad,
                         ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_a_plane, planes,
@@ -151,12 +154,25 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         SplitOperand sb{ws + d.off_sb, ceil_div(d.N, kSplitRowPad) * kSplitRowPad,
                         ceil_div(d.K, kSplitKPad) * kSplitKPad, d.batch, d.split_b_plane, planes,
                         planes == 4 ? 1 : 0};
-        launch_split_bf16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
-        launch_split_bf16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
-        if (planes == 6)
-            launch_gemm_bf16x3(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
-        else
-            launch_gemm_bf16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
+        if (d.gemm_kernel == kGemmF16x4m) {
+            sa.rowmax = reinterpret_cast<uint32_t *>(ws + d.off_rmax_a);
+            sa.scale = reinterpret_cast<float *>(ws + d.off_scale_a);
+            sb.rowmax = reinterpret_cast<uint32_t *>(ws + d.off_rmax_b);
+            sb.scale = reinterpret_cast<float *>(ws + d.off_scale_b);
+            launch_split_f16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
+            launch_split_f16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
+            launch_gemm_f16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
+        }
+        else {
+            launch_split_bf16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
+            launch_split_bf16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
+            if (planes == 6)
+                launch_gemm_bf16x3(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false,
+                                   false, s);
+            else
+                launch_gemm_bf16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false,
+                                    false, s);
+        }
     }
     else {
         GemmArgs g;

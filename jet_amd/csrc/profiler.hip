@@ -26,7 +26,8 @@ double g_ms[kNumKernels], g_flops[kNumKernels], g_bytes[kNumKernels];
 const char *const kNames[kNumKernels] = {
     "permute_bits", "permute_generic", "copy",       "gemm_valu",    "gemm_valu_reduce",
     "gemm_fp32_mfma", "gemm_fp64_mfma", "split_bf16", "gemm_bf16x3_mfma", "add",
-    "fill_random",  "reduce_compensated", "convert", "gemm_splitk", "gemm_bf16x3_4m_mfma"};
+    "fill_random",  "reduce_compensated", "convert", "gemm_splitk", "gemm_bf16x3_4m_mfma",
+    "split_f16",    "gemm_f16x3_4m_mfma"};
 
 hipEvent_t take_event()
 {

@@ -28,6 +28,8 @@ enum Kernel {
     kConvert,
     kGemmSplitK,
     kGemmBf16x4m,
+    kSplitF16,
+    kGemmF16x4m,
     kNumKernels
 };
 

@@ -35,7 +35,8 @@ import numpy as np
 from . import _native as nat
 
 _PREC = {None: nat.PREC_DEFAULT, "default": nat.PREC_DEFAULT, "fp32": nat.PREC_FP32,
-         "bf16x3": nat.PREC_BF16X3, "fp64": nat.PREC_FP64, "bf16x3_4m": nat.PREC_BF16X3_4M}
+         "bf16x3": nat.PREC_BF16X3, "fp64": nat.PREC_FP64, "bf16x3_4m": nat.PREC_BF16X3_4M,
+         "f16x3": nat.PREC_F16X3_4M}
 
 
 class LabelIds:

@@ -102,6 +102,20 @@ class TensorNetwork:
     NumIndices = lambda self: len(self._edges)  # noqa: E731
     NumTensors = lambda self: len(self._nodes)  # noqa: E731
 
+    def copy(self) -> "TensorNetwork":
+        """The copy constructor: node, edge and tag records are copied, the
+        tensors shared (slicing and contraction replace a node's tensor,
+        they never modify it)."""
+        out = TensorNetwork(self.tensor_type, self.dtype)
+        out._nodes = [Node(n.id, n.name, list(n.indices), list(n.tags), n.contracted, n.tensor)
+                      for n in self._nodes]
+        out._edges = {k: Edge(e.dim, list(e.node_ids)) for k, e in self._edges.items()}
+        out._tags = {k: list(v) for k, v in self._tags.items()}
+        out._path = list(self._path)
+        return out
+
+    __copy__ = copy
+
     # --- building ------------------------------------------------------------
     def add_tensor(self, tensor, tags: Optional[Sequence[str]] = None) -> int:
         """TensorNetwork::AddTensor (TensorNetwork.hpp:164-180)."""
@@ -148,8 +162,9 @@ class TensorNetwork:
                 tensor = T.SliceIndex(tensor, label, v)
                 if list(tensor.indices) != keep:
                     tensor = T.Transpose(tensor, keep)
-                if keep:
+                if keep and list(tensor.shape) != keep_shape:
                     tensor = T.Reshape(tensor, keep_shape)
+                # relabels in place: `tensor` is a new tensor made above
                 tensor.init_indices_and_shape(keep, keep_shape)
                 node.indices = [f"{l}({v})" if l == label else l for l in node.indices]
             node.tensor = tensor

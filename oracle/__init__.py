@@ -388,3 +388,52 @@ def baseline_contract(a_labels, a_shape, a, b_labels, b_shape, b, nthreads: int 
     if err:
         raise OracleError(_ERRORS.get(err, str(err)))
     return left + right, C
+
+
+# ---------------------------------------------------------------------------
+# Environments (adjoint contraction) in complex128.  The reference has no
+# gradient code; this is the CPU reference gradient the GPU adjoint is checked
+# against.  It is pinned in tests by the exact derivative of a statevector
+# simulation (the circuit with one gate replaced by its derivative matrix).
+# ---------------------------------------------------------------------------
+
+def environments(tensors: List[Tuple[Sequence[str], Sequence[int], np.ndarray]],
+                 path: Sequence[Tuple[int, int]], wanted: Sequence[int],
+                 nthreads: Optional[int] = None):
+    """Returns (A, {leaf: E_leaf}) for a closed network: A the full
+    contraction along the SSA path, E_leaf = dA/d(leaf) as complex128 data in
+    the leaf's own index order."""
+    n = len(tensors)
+    nodes = [(list(l), list(s), np.asarray(d, dtype=np.complex128).reshape(-1))
+             for (l, s, d) in tensors]
+    children = {}
+    for i, (a, b) in enumerate(path):
+        children[n + i] = (a, b)
+        la, sa, da = nodes[a]
+        lb, sb, db = nodes[b]
+        nodes.append(contract(la, sa, da, lb, sb, db, nthreads))
+    root = n + len(path) - 1
+    need = set(wanted)
+    for i, (a, b) in enumerate(path):
+        if a in need or b in need:
+            need.add(n + i)
+    env = {root: ([], [], np.ones(1, np.complex128))}
+    out = {}
+    for i in range(len(path) - 1, -1, -1):
+        z = n + i
+        if z not in env:
+            continue
+        lz, sz, ez = env.pop(z)
+        for (u, v) in (children[z], children[z][::-1]):
+            if u not in need:
+                continue
+            lv, sv, dv = nodes[v]
+            le, se, de = contract(lz, sz, ez, lv, sv, dv, nthreads)
+            lu, su, _ = nodes[u]
+            perm = [le.index(l) for l in lu]
+            de = np.ascontiguousarray(de.reshape(se).transpose(perm)).reshape(-1) if se else de
+            if u < n:
+                out[u] = de
+            else:
+                env[u] = (lu, su, de)
+    return complex(nodes[root][2][0]), out

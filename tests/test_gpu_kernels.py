@@ -19,7 +19,7 @@ from hiputil import contract, permute, rand_c, rel_err, say
 pytestmark = pytest.mark.gpu
 
 TOL = {nat.PREC_FP32: 1e-5, nat.PREC_BF16X3: 1e-5, nat.PREC_FP64: 1e-12,
-       nat.PREC_DEFAULT: 1e-5, nat.PREC_BF16X3_4M: 1e-5}
+       nat.PREC_DEFAULT: 1e-5, nat.PREC_BF16X3_4M: 1e-5, nat.PREC_F16X3_4M: 2e-6}
 
 
 # --------------------------------------------------------------------------
@@ -134,6 +134,7 @@ def _random_problem(rng, max_rank=6, dims=(1, 2, 3, 4)):
 @pytest.mark.parametrize("prec,dtype", [(nat.PREC_FP32, np.complex64),
                                         (nat.PREC_BF16X3, np.complex64),
                                         (nat.PREC_BF16X3_4M, np.complex64),
+                                        (nat.PREC_F16X3_4M, np.complex64),
                                         (nat.PREC_FP64, np.complex128)])
 def test_contract_random_small(ctx, prec, dtype):
     rng = np.random.default_rng(100 + prec)
@@ -158,6 +159,9 @@ def test_contract_random_small(ctx, prec, dtype):
     (nat.PREC_BF16X3_4M, np.complex64, (512, 512, 512)),
     (nat.PREC_BF16X3_4M, np.complex64, (300, 200, 170)),
     (nat.PREC_BF16X3_4M, np.complex64, (1000, 260, 1100)),
+    (nat.PREC_F16X3_4M, np.complex64, (512, 512, 512)),
+    (nat.PREC_F16X3_4M, np.complex64, (300, 200, 170)),
+    (nat.PREC_F16X3_4M, np.complex64, (1000, 260, 1100)),
     (nat.PREC_FP64, np.complex128, (256, 128, 96)),
     (nat.PREC_FP64, np.complex128, (130, 70, 50)),
     (nat.PREC_FP32, np.complex64, (8, 4, 4000)),
@@ -183,6 +187,8 @@ def test_gemm_kernels(ctx, prec, dtype, shape):
         assert info["kernel"] == "bf16x3_mfma"
     if prec == nat.PREC_BF16X3_4M and min(shape) >= 128:
         assert info["kernel"] == "bf16x3_4m_mfma"
+    if prec == nat.PREC_F16X3_4M and min(shape) >= 128:
+        assert info["kernel"] == "f16x3_4m_mfma"
 
 
 def test_batch_modes(ctx):
@@ -311,7 +317,8 @@ def test_reduce_compensated(ctx):
     assert abs(got[0] - want.real) < 1e-6 and abs(got[1] - want.imag) < 1e-6
 
 
-@pytest.mark.parametrize("prec", [nat.PREC_BF16X3, nat.PREC_BF16X3_4M, nat.PREC_FP32])
+@pytest.mark.parametrize("prec", [nat.PREC_BF16X3, nat.PREC_BF16X3_4M, nat.PREC_F16X3_4M,
+                                  nat.PREC_FP32])
 def test_batch_modes_mfma(ctx, prec):
     """Batched MFMA GEMMs (batch modes in A, B and C) at tile-sized shapes."""
     rng = np.random.default_rng(13)
@@ -325,3 +332,25 @@ def test_batch_modes_mfma(ctx, prec):
                      b.reshape(K, S, N).astype(np.complex128)).reshape(-1)
     assert info["batch"] == S
     assert rel_err(got, want) < 1e-5, (info, rel_err(got, want))
+
+
+def test_split_f16_row_scales(ctx):
+    """Split-FP16 scales every operand row by its own power of two: rows whose
+    magnitudes differ by up to 10^24 keep FP32-level relative accuracy
+    (error measured per output against |A row| * |B row| * sqrt(K)), where an
+    unscaled FP16 split would overflow or flush them."""
+    rng = np.random.default_rng(21)
+    M, N, K = 256, 192, 320
+    sa = 10.0 ** rng.uniform(-12, 12, M)
+    sb = 10.0 ** rng.uniform(-12, 12, N)
+    a = (rand_c(rng, M * K, np.complex128).reshape(M, K) * sa[:, None]).astype(np.complex64)
+    b = (rand_c(rng, N * K, np.complex128).reshape(N, K) * sb[:, None]).astype(np.complex64)
+    say("split-f16 row scales")
+    # A(m,k) . B(n,k) -> C(m,n)
+    got, info = contract(ctx, a.reshape(-1), [0, 1], [M, K], b.reshape(-1), [2, 1], [N, K],
+                         [0, 2], precision=nat.PREC_F16X3_4M)
+    assert info["kernel"] == "f16x3_4m_mfma", info
+    want = a.astype(np.complex128) @ b.astype(np.complex128).T
+    norm = np.abs(a).max(1)[:, None] * np.abs(b).max(1)[None, :] * np.sqrt(K)
+    err = np.abs(got.reshape(M, N) - want) / norm
+    assert err.max() < 1e-6, err.max()
