@@ -10,16 +10,19 @@ Workloads:
       sliced on 12 bond-2 indices (4096 slices).  The slice loop runs on the
       GPU in batches: 2^b slices per execution (b sliced indices kept as
       batch modes), executions over the other sliced indices.  A step is one
-      execution (2^b slices); the timed region runs --steps executions in
-      split-BF16 3M, then the same in plain FP32 MFMA; one execution in
-      complex128 on FP64 MFMA is the FP64 reference of the relative error.
-      Each execution's per-slice amplitudes are summed by the FP64
-      compensated reduction.
+      execution (2^b slices); the timed region runs --steps executions with
+      the split 16-bit MFMA GEMM of --split (default split-FP16: each
+      complex64 operand row scaled by a power of two, split into hi + lo
+      FP16, three products per real product, FP32 accumulation), then the
+      --compare modes (default split-BF16) and plain FP32 MFMA; one
+      execution in complex128 on FP64 MFMA is the FP64 reference of the
+      relative errors.  Each execution's per-slice amplitudes are summed by
+      the FP64 compensated reduction.
   c1  single large pairwise contraction: complex64 A, B of rank 28 (all
       extents 2, 2^28 elements each), 14 shared indices at seeded random
       positions in both index orders, re/im ~ N(0, 1/2); one
       2^14 x 2^14 x 2^14 complex GEMM behind the index permutations.
-      A step is one ContractTensors call in split-BF16 3M (the headline
+      A step is one ContractTensors call with the --split GEMM (the headline
       value); the plain FP32-MFMA path is timed beside it, and complex128 on
       FP64 MFMA gives the FP64 reference the error is measured against.
 
@@ -544,9 +547,10 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--batch-log2", type=int, default=2,
                     help="c2: sliced indices batched per execution (2^b slices per step)")
-    ap.add_argument("--split", default="4m", choices=sorted(SPLIT_MODES),
-                    help="complex form of the split-BF16 GEMM")
-    ap.add_argument("--compare", nargs="*", default=["4m", "f16"], choices=sorted(SPLIT_MODES),
+    ap.add_argument("--split", default="f16", choices=sorted(SPLIT_MODES),
+                    help="split 16-bit MFMA GEMM: f16 (split-FP16 4M), 4m (split-BF16 "
+                         "4M), 3m (split-BF16 3M)")
+    ap.add_argument("--compare", nargs="*", default=["4m"], choices=sorted(SPLIT_MODES),
                     help="c2: other split modes timed for comparison")
     ap.add_argument("--steps32", type=int, default=1,
                     help="c2: executions timed in plain FP32")
