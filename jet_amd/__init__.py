@@ -12,8 +12,11 @@ from .permuter import Permuter
 from .path_info import PathInfo, PathStepInfo
 from .tensor_network import TensorNetwork
 from .task_based_contractor import TaskBasedContractor
+from .tensor_network_io import TensorFileException, TensorNetworkFile, TensorNetworkSerializer
 from .factory import (TensorC64, TensorC128, TensorNetworkC64, TensorNetworkC128,
-                      TaskBasedContractorC64, TaskBasedContractorC128)
+                      TaskBasedContractorC64, TaskBasedContractorC128, TensorNetworkFileC64,
+                      TensorNetworkFileC128, TensorNetworkSerializerC64,
+                      TensorNetworkSerializerC128)
 from . import factory, utilities
 
 __version__ = "0.1.0"

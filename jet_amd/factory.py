@@ -2,9 +2,9 @@
 
 python/jet/factory.py:40-151 and python/src/Type.hpp:10-18 expose
 ``TensorC64``/``TensorC128``, ``TensorNetworkC64``/``C128`` and
-``TaskBasedContractorC64``/``C128``, plus dtype-dispatching ``Tensor``,
-``TensorNetwork`` and ``TaskBasedContractor`` factories whose default
-dtype is complex128.  The same names here build the GPU-backed classes.
+``TaskBasedContractorC64``/``C128``, ``TensorNetworkFileC64``/``C128`` and
+``TensorNetworkSerializerC64``/``C128``, plus dtype-dispatching factories of
+the same names without the suffix, whose default dtype is complex128.  The same names here build the GPU-backed classes.
 """
 from __future__ import annotations
 
@@ -13,10 +13,14 @@ import numpy as np
 from .task_based_contractor import TaskBasedContractor as _TBC
 from .tensor import Tensor as _Tensor
 from .tensor_network import TensorNetwork as _TN
+from .tensor_network_io import TensorNetworkFile as _TNF
+from .tensor_network_io import TensorNetworkSerializer as _TNS
 
 __all__ = ["TensorC64", "TensorC128", "TensorNetworkC64", "TensorNetworkC128",
            "TaskBasedContractorC64", "TaskBasedContractorC128", "Tensor", "TensorNetwork",
-           "TaskBasedContractor"]
+           "TaskBasedContractor", "TensorNetworkFileC64", "TensorNetworkFileC128",
+           "TensorNetworkSerializerC64", "TensorNetworkSerializerC128", "TensorNetworkFile",
+           "TensorNetworkSerializer"]
 
 
 def _check(dtype) -> np.dtype:
@@ -60,3 +64,27 @@ def TensorNetwork(dtype=np.complex128) -> _TN:
 
 def TaskBasedContractor(*args, dtype=np.complex128, **kwargs) -> _TBC:
     return _TBC(*args, tensor_type=_Tensor, dtype=_check(dtype), **kwargs)
+
+
+def TensorNetworkFileC64() -> _TNF:
+    return _TNF(_TN(_Tensor, np.complex64))
+
+
+def TensorNetworkFileC128() -> _TNF:
+    return _TNF(_TN(_Tensor, np.complex128))
+
+
+def TensorNetworkSerializerC64(indent: int = -1) -> _TNS:
+    return _TNS(indent, _Tensor, np.complex64)
+
+
+def TensorNetworkSerializerC128(indent: int = -1) -> _TNS:
+    return _TNS(indent, _Tensor, np.complex128)
+
+
+def TensorNetworkFile(dtype=np.complex128) -> _TNF:
+    return _TNF(_TN(_Tensor, _check(dtype)))
+
+
+def TensorNetworkSerializer(indent: int = -1, dtype=np.complex128) -> _TNS:
+    return _TNS(indent, _Tensor, _check(dtype))
