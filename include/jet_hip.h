@@ -80,6 +80,7 @@ typedef enum {
 typedef struct jet_hip_ctx_s *jet_hip_ctx_t;
 typedef struct jet_hip_plan_s *jet_hip_plan_t;
 typedef struct jet_hip_network_s *jet_hip_network_t;
+typedef struct jet_hip_batch_s *jet_hip_batch_t;
 
 /* ---- library / errors -------------------------------------------------- */
 
@@ -240,6 +241,21 @@ int jet_hip_network_capture(jet_hip_network_t net, const void *const *leaves,
                             void *out, void *stream);
 int jet_hip_network_launch(jet_hip_network_t net, void *stream);
 int jet_hip_network_destroy(jet_hip_network_t net);
+
+/* ---- uneven batches: executions of several planned networks (different
+ *      circuits, bitstrings, slice ranges) captured as one hipGraph and
+ *      launched at once.  Replaces the composition of many contraction
+ *      task graphs into one executor run (TaskBasedContractor::AddTaskflow,
+ *      include/jet/TaskBasedContractor.hpp:147-155, and the cudaFlow of
+ *      examples/heterogeneous_contraction.cu:91-142).
+ * Execution i runs networks[i] on leaves[i] (that network's leaf pointers)
+ * into outs[i].  A network may appear several times: its executions run in
+ * the order given and share its arena.  All networks belong to ctx.      */
+int jet_hip_batch_capture(jet_hip_ctx_t ctx, int n, const jet_hip_network_t *networks,
+                          const void *const *const *leaves, void *const *outs,
+                          void *stream, jet_hip_batch_t *batch);
+int jet_hip_batch_launch(jet_hip_batch_t batch, void *stream);
+int jet_hip_batch_destroy(jet_hip_batch_t batch);
 
 /* ---- dtype conversion (Tensor<complex<float>> <-> Tensor<complex<double>>
  *      data, as used to build the FP64 oracle inputs on the device) ------- */

@@ -42,7 +42,8 @@ EXPORTS = [
     "jet_hip_profiler_enable", "jet_hip_profiler_reset", "jet_hip_profiler_read",
     "jet_hip_profiler_kernel_name", "jet_hip_network_create", "jet_hip_network_estimate", "jet_hip_network_info",
     "jet_hip_network_step_info", "jet_hip_network_execute", "jet_hip_network_capture",
-    "jet_hip_network_launch", "jet_hip_network_destroy",
+    "jet_hip_network_launch", "jet_hip_network_destroy", "jet_hip_batch_capture",
+    "jet_hip_batch_launch", "jet_hip_batch_destroy",
 ]
 
 
@@ -115,6 +116,11 @@ def _declare(L):
         "jet_hip_network_capture": [_vp, ctypes.POINTER(_vp), _vp, _vp],
         "jet_hip_network_launch": [_vp, _vp],
         "jet_hip_network_destroy": [_vp],
+        "jet_hip_batch_capture": [_vp, ctypes.c_int, ctypes.POINTER(_vp),
+                                  ctypes.POINTER(ctypes.POINTER(_vp)), ctypes.POINTER(_vp), _vp,
+                                  ctypes.POINTER(_vp)],
+        "jet_hip_batch_launch": [_vp, _vp],
+        "jet_hip_batch_destroy": [_vp],
         "jet_hip_profiler_enable": [ctypes.c_int],
         "jet_hip_profiler_reset": [],
         "jet_hip_profiler_read": [ctypes.c_int, _i64p, ctypes.POINTER(ctypes.c_double),

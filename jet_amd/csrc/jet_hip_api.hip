@@ -38,6 +38,12 @@ struct jet_hip_network_s {
     hipGraphExec_t graph_exec = nullptr;
 };
 
+struct jet_hip_batch_s {
+    jet_hip_ctx_s *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -730,6 +736,73 @@ int jet_hip_network_launch(jet_hip_network_t net, void *stream)
         JH_REQUIRE(net && net->graph_exec, "The network has no captured graph.");
         set_device(net->ctx);
         JH_HIP(hipGraphLaunch(net->graph_exec, pick_stream(net->ctx, stream)));
+    });
+}
+
+int jet_hip_batch_capture(jet_hip_ctx_t ctx, int n, const jet_hip_network_t *networks,
+                          const void *const *const *leaves, void *const *outs, void *stream,
+                          jet_hip_batch_t *batch)
+{
+    return guarded([&] {
+        JH_REQUIRE(ctx && batch && n >= 1 && networks && leaves && outs, "Null argument.");
+        for (int i = 0; i < n; i++) {
+            JH_REQUIRE(networks[i] && leaves[i] && outs[i], "Null network, leaves or output.");
+            JH_REQUIRE(networks[i]->ctx == ctx, "Every network of a batch needs the batch's context.");
+        }
+        set_device(ctx);
+        hipStream_t s = pick_stream(ctx, stream);
+        auto *b = new jet_hip_batch_s;
+        b->ctx = ctx;
+        const bool prof_on = prof::enabled();
+        prof::set_enabled(false);
+        hipError_t st = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (st != hipSuccess) {
+            prof::set_enabled(prof_on);
+            delete b;
+            JH_HIP(st);
+        }
+        try {
+            for (int i = 0; i < n; i++) run_network(networks[i], leaves[i], outs[i], s);
+        }
+        catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            prof::set_enabled(prof_on);
+            delete b;
+            throw;
+        }
+        prof::set_enabled(prof_on);
+        st = hipStreamEndCapture(s, &b->graph);
+        if (st == hipSuccess)
+            st = hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0);
+        if (st != hipSuccess) {
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            delete b;
+            JH_HIP(st);
+        }
+        *batch = b;
+    });
+}
+
+int jet_hip_batch_launch(jet_hip_batch_t batch, void *stream)
+{
+    return guarded([&] {
+        JH_REQUIRE(batch && batch->graph_exec, "Null batch.");
+        set_device(batch->ctx);
+        JH_HIP(hipGraphLaunch(batch->graph_exec, pick_stream(batch->ctx, stream)));
+    });
+}
+
+int jet_hip_batch_destroy(jet_hip_batch_t batch)
+{
+    return guarded([&] {
+        if (!batch) return;
+        set_device(batch->ctx);
+        JH_HIP(hipDeviceSynchronize());
+        if (batch->graph_exec) (void)hipGraphExecDestroy(batch->graph_exec);
+        if (batch->graph) (void)hipGraphDestroy(batch->graph);
+        delete batch;
     });
 }
 

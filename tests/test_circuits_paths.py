@@ -135,3 +135,26 @@ def test_choose_batch_respects_budget():
     assert choose_batch(net.indices, net.shapes, path, sliced, 1e12) == sliced
     small = choose_batch(net.indices, net.shapes, path, sliced, 64 * 1024)
     assert len(small) < len(sliced)
+
+
+@pytest.mark.parametrize("mode", [0, 4])
+def test_stacked_amplitude_plan(mode):
+    """The stacked form of an amplitude batch (a bitstring label on every
+    output vector, kept open): the plan stays inside its arena and its
+    flops are at least those of the single-amplitude plan."""
+    L = _check_lib()
+    circ = circuits.random_grid_circuit(4, 4, 8, seed=4)
+    net = circuits.amplitude_network(circ)
+    dims = {l: 2 for ix in net.indices for l in ix}
+    path = paths.best_minfill_path(net.indices, dims, trials=2, seed=1)
+    idx = [list(ix) for ix in net.indices]
+    shp = [list(s) for s in net.shapes]
+    nq = circ.n_qubits
+    for q in range(nq):
+        k = len(idx) - nq + q
+        idx[k] = ["%amplitude"] + idx[k]
+        shp[k] = [6] + shp[k]
+    rc, fl, ar, ws, err = _check(L, idx, shp, path, ["%amplitude"], mode=mode)
+    assert rc == 0, err
+    rc1, fl1, *_ = _check(L, net.indices, net.shapes, path, [], mode=mode)
+    assert rc1 == 0 and fl >= fl1
