@@ -539,12 +539,107 @@ def c2_cpu_baseline(host, threads, budget_s=20.0):
                       f"per-slice flops"}
 
 
+C3_CIRCUITS = [((5, 5, 14), 24, 51), ((6, 6, 12), 16, 52), ((6, 7, 10), 24, 53)]
+
+
+def run_c3(args):
+    """Uneven batch: 64 amplitudes of three seeded grid circuits (5x5 depth
+    14, 6x6 depth 12, 6x7 depth 10; bitstrings seeded) plus a range of
+    --c3-slices slices of the c2 amplitude, all executions captured in one
+    hipGraph (jet_amd.batch.UnevenBatch; each amplitude job's form, one
+    execution per bitstring or all bitstrings batched, chosen by timing).
+    One launch is timed with the --split GEMM; the same batch in complex128
+    on FP64 MFMA (the c2 part: its first execution only) gives the
+    per-amplitude errors."""
+    from jet_amd import _native as nat, circuits, paths
+    from jet_amd.batch import AmplitudeJob, SliceJob, UnevenBatch
+
+    ctx = nat.default_context()
+    rng = np.random.default_rng(2026)
+    jobs = []
+    for (r, c, d), n_amp, seed in C3_CIRCUITS:
+        circ = circuits.random_grid_circuit(r, c, d, seed=seed)
+        net = circuits.amplitude_network(circ)
+        dims = {l: 2 for ix in net.indices for l in ix}
+        path = paths.best_minfill_path(net.indices, dims, trials=8, seed=1)
+        bits = [tuple(int(x) for x in rng.integers(0, 2, r * c)) for _ in range(n_amp)]
+        jobs.append(AmplitudeJob(circ, bits, path))
+    plan, circ2, net2 = c2_problem()
+    sliced = plan["sliced"]
+    batch = sliced[len(sliced) - args.batch_log2:] if args.batch_log2 else []
+    per_exec = 2 ** len(batch)
+    n_exec = max(1, args.c3_slices // per_exec)
+    data32 = [d.astype(np.complex64) for d in net2.data]
+    jobs.append(SliceJob(net2.indices, net2.shapes, data32, plan["path"], sliced, batch, 0,
+                         n_exec))
+    n_amp = sum(len(j.bitstrings) for j in jobs[:-1])
+    split = SPLIT_MODES[args.split][1]
+    log(f"c3: {n_amp} amplitudes over {len(jobs) - 1} circuits + {n_exec * per_exec} c2 slices")
+
+    t0 = time.perf_counter()
+    ub = UnevenBatch(jobs, dtype=np.complex64, precision=split, ctx=ctx)
+    log(f"c3: batch captured ({time.perf_counter() - t0:.1f}s), forms {ub.forms()}")
+    flops = 0.0
+    for job, part in zip(ub.jobs, ub.parts):
+        if isinstance(job, SliceJob):
+            flops += part.net.flops * job.executions
+        else:
+            flops += part.net.flops * (len(part.execs))
+    e0, e1 = nat.Event(ctx), nat.Event(ctx)
+    ctx.synchronize()
+    e0.record()
+    ub.launch()
+    e1.record()
+    ctx.synchronize()
+    ms = e0.elapsed_ms(e1)
+    amps = [ub.amplitudes(j) for j in range(len(jobs) - 1)]
+    slices = ub.slice_results(len(jobs) - 1)
+    log(f"c3: one launch {ms:.1f} ms, {flops / (ms * 1e-3) / 1e12:.1f} TFLOP/s")
+    forms, tuning = ub.forms(), ub.tuning
+    del ub
+
+    ref_jobs = jobs[:-1] + [SliceJob(net2.indices, net2.shapes, net2.data, plan["path"],
+                                     sliced, batch, 0, 1)]
+    ub64 = UnevenBatch(ref_jobs, dtype=np.complex128, precision="fp64", ctx=ctx)
+    ub64.launch()
+    ctx.synchronize()
+    errs = []
+    for j, a in enumerate(amps):
+        ref = ub64.amplitudes(j)
+        errs.extend((np.abs(a - ref) / np.abs(ref)).tolist())
+    ref_s = ub64.slice_results(len(jobs) - 1)
+    slice_err = float(np.abs(slices[:per_exec] - ref_s).max() / np.abs(ref_s).max())
+    errs = np.array(errs)
+    log(f"c3: per-amplitude rel err max {errs.max():.2e} median {np.median(errs):.2e}; "
+        f"c2 slices {slice_err:.2e}")
+    return {
+        "metric": METRIC,
+        "value": round(flops / (ms * 1e-3) / 1e12, 3),
+        "unit": "TFLOP/s",
+        "ms_per_step": round(ms, 3),
+        "dtype": f"c64 ({SPLIT_LABEL[args.split]} MFMA on MFMA-bound steps, fp32 MFMA elsewhere)",
+        "config": {"workload": "c3: uneven batch in one hipGraph launch: 24 amplitudes of a "
+                               "5x5 depth-14, 16 of a 6x6 depth-12 and 24 of a 6x7 depth-10 "
+                               f"seeded grid circuit, plus {n_exec * per_exec} slices of the "
+                               "c2 7x7 depth-14 amplitude; a step is one launch",
+                   "amplitudes": n_amp, "c2_slices": n_exec * per_exec},
+        "forms": forms,
+        "tuning_ms": tuning,
+        "rel_err_vs_fp64": {"amplitude_max": float(errs.max()),
+                            "amplitude_median": float(np.median(errs)),
+                            "per_amplitude": [float(x) for x in errs],
+                            "c2_slices_first_execution": slice_err},
+    }, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--c3-slices", type=int, default=256,
+                    help="c3: slices of the c2 amplitude in the batch")
     ap.add_argument("--batch-log2", type=int, default=2,
                     help="c2: sliced indices batched per execution (2^b slices per step)")
     ap.add_argument("--split", default="f16", choices=sorted(SPLIT_MODES),
@@ -570,6 +665,9 @@ def main():
         if args.quick:
             print(json.dumps(result), flush=True)
             return
+    elif args.workload == "c3":
+        result, host = run_c3(args)
+        args.no_cpu_baseline = True
     else:
         result, host = run_c2(args)
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
