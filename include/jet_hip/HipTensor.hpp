@@ -34,8 +34,11 @@
 #include <utility>
 #include <vector>
 
-#ifndef JET_ABORT
-// Jet's Abort.hpp was not included: provide the same exception type.
+#if __has_include("jet/Abort.hpp")
+// Jet's own exception type, whichever header comes first.
+#include "jet/Abort.hpp"
+#else
+// Without Jet's headers: the same exception type.
 namespace Jet {
 class Exception : public std::exception {
   public:

@@ -51,3 +51,33 @@ def test_hip_tensor_reference_cases():
     msg = buf.value.decode()
     print(msg, flush=True)
     assert rc == 0, msg
+
+
+REF_NET = os.path.join(ROOT, "oracle", "_ref", "libref_network.so")
+
+
+def _ref_network_lib():
+    if not os.path.exists(REF_NET):
+        pytest.skip("oracle/_ref/libref_network.so not built (needs the reference tree)")
+    _native.lib()
+    lib = ctypes.CDLL(REF_NET)
+    lib.jr_network_hiptensor_check.restype = ctypes.c_int
+    lib.jr_network_hiptensor_check.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    return lib
+
+
+def test_reference_network_templates_library_loads():
+    assert hasattr(_ref_network_lib(), "jr_network_hiptensor_check")
+
+
+@pytest.mark.gpu
+def test_reference_network_templates_over_hip_tensor():
+    """The reference's own TensorNetwork / PathInfo templates
+    (TensorNetwork.hpp, PathInfo.hpp) instantiated with HipTensor reproduce
+    the values the reference's tests expect of TensorNetwork<CudaTensor>."""
+    lib = _ref_network_lib()
+    buf = ctypes.create_string_buffer(1024)
+    rc = lib.jr_network_hiptensor_check(buf, len(buf))
+    msg = buf.value.decode()
+    print(msg, flush=True)
+    assert rc == 0, msg
