@@ -394,14 +394,20 @@ def minfill_path(indices: Sequence[Sequence[str]], dims: Dict[str, int], seed: i
     return path
 
 
+def minfill_trial(indices, dims, t: int, seed: int = 0, open_indices=()) -> Path:
+    """Trial t of the min-fill search: t = 0 deterministic, then randomised
+    elimination orders with noise levels cycling through 0.3 .. 1.2."""
+    return minfill_path(indices, dims, seed=seed + t,
+                        noise=0.0 if t == 0 else 0.3 + 0.1 * (t % 10),
+                        open_indices=open_indices)
+
+
 def best_minfill_path(indices, dims, trials: int = 16, seed: int = 0,
                       open_indices=()) -> Path:
     """Deterministic min-fill plus randomised trials; fewest flops wins."""
     best, best_key = None, None
     for t in range(trials):
-        p = minfill_path(indices, dims, seed=seed + t,
-                         noise=0.0 if t == 0 else 0.3 + 0.1 * (t % 10),
-                         open_indices=open_indices)
+        p = minfill_trial(indices, dims, t, seed, open_indices)
         st = path_stats(indices, dims, p, open_indices=open_indices)
         key = (st.total_flops, st.max_log2_size)
         if best_key is None or key < best_key:
@@ -442,17 +448,44 @@ def choose_slices_for_flops(indices, dims, path: Path, n_slices: int,
     return chosen
 
 
+def _sliced_trial(args):
+    indices, dims, t, seed, n_slices, max_log2_size = args
+    p = minfill_trial(indices, dims, t, seed)
+    sl = choose_slices_for_flops(indices, dims, p, n_slices, max_log2_size)
+    per = path_stats(indices, dims, p, sliced=sl)
+    return t, per.total_flops, per.max_log2_size, p, sl
+
+
+def best_sliced_minfill_plan(indices, dims, n_slices: int, max_log2_size: float,
+                             trials: int = 64, seed: int = 0, workers: int = 1):
+    """Min-fill trials judged by their sliced cost: each trial's path is
+    sliced on `n_slices` indices (choose_slices_for_flops) and the trial
+    with the fewest per-slice flops whose largest per-slice intermediate
+    fits max_log2_size wins (the unsliced flops of a path predict its sliced
+    cost poorly).  Returns (path, sliced indices)."""
+    jobs = [(indices, dims, t, seed, n_slices, max_log2_size) for t in range(trials)]
+    if workers > 1:
+        from multiprocessing import Pool
+        with Pool(workers) as pool:
+            results = pool.map(_sliced_trial, jobs)
+    else:
+        results = [_sliced_trial(j) for j in jobs]
+    ok = [r for r in results if r[2] <= max_log2_size] or results
+    best = min(ok, key=lambda r: (r[1], r[0]))
+    return best[3], best[4]
+
+
 def grid_circuit_plan(rows: int, cols: int, depth: int, seed: int, n_slices: int,
-                      trials: int = 16, max_log2_size: float = 29.0) -> dict:
+                      trials: int = 16, max_log2_size: float = 29.0, workers: int = 1) -> dict:
     """Contraction path and sliced indices for one amplitude of a seeded grid
-    random circuit (jet_amd.circuits): best min-fill path of `trials`, then
-    `n_slices` indices chosen for per-slice flops."""
+    random circuit (jet_amd.circuits): of `trials` min-fill paths, the one
+    with the fewest per-slice flops after choosing `n_slices` indices."""
     from . import circuits
     circ = circuits.random_grid_circuit(rows, cols, depth, seed=seed)
     net = circuits.amplitude_network(circ)
     dims = {l: 2 for ix in net.indices for l in ix}
-    path = best_minfill_path(net.indices, dims, trials=trials, seed=seed)
-    sliced = choose_slices_for_flops(net.indices, dims, path, n_slices, max_log2_size)
+    path, sliced = best_sliced_minfill_plan(net.indices, dims, n_slices, max_log2_size,
+                                            trials, seed, workers)
     full = path_stats(net.indices, dims, path)
     per = path_stats(net.indices, dims, path, sliced=sliced)
     return {"rows": rows, "cols": cols, "depth": depth, "seed": seed, "trials": trials,
@@ -494,8 +527,10 @@ if __name__ == "__main__":
     ap.add_argument("--slices", type=int, default=12)
     ap.add_argument("--trials", type=int, default=64)
     ap.add_argument("--max-log2-size", type=float, default=29.0)
+    ap.add_argument("--workers", type=int, default=1)
     a = ap.parse_args()
-    plan = grid_circuit_plan(a.rows, a.cols, a.depth, a.seed, a.slices, a.trials, a.max_log2_size)
+    plan = grid_circuit_plan(a.rows, a.cols, a.depth, a.seed, a.slices, a.trials, a.max_log2_size,
+                             a.workers)
     d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
     os.makedirs(d, exist_ok=True)
     out = os.path.join(d, f"grid_{a.rows}x{a.cols}x{a.depth}_seed{a.seed}_s{a.slices}.json")
