@@ -355,26 +355,34 @@ def run_c2(args):
 
     def build(prec, dtype):
         data = [d.astype(dtype) for d in net.data]
-        # Eager launches: each execution is seconds of GPU work, so launch
-        # overhead is negligible, and every kernel is then visible to the
-        # event profiler used for the roofline.
         return SlicedContraction(net.indices, net.shapes, data, path, sliced, batch=batch,
                                  dtype=dtype, precision=prec, ctx=ctx, use_graph=False)
 
     def timed(sc, iters, warmup, profile):
-        sc.run(range(warmup))
+        """The timed executions run as one captured hipGraph launch (the
+        warm-up executions as another, launched first).  With `profile`, one
+        more execution runs eagerly under the event profiler afterwards: the
+        per-kernel times of the roofline."""
+        if warmup:
+            wb = sc.capture(range(warmup))
+            wb.launch()
+            ctx.synchronize()
+            wb.close()
+        tb = sc.capture(range(warmup, warmup + iters))
         ctx.synchronize()
-        if profile:
-            nat.Profiler.reset()
-            nat.Profiler.enable(True)
         e0, e1 = nat.Event(ctx), nat.Event(ctx)
         e0.record()
-        sc.run(range(warmup, warmup + iters))
+        tb.launch()
         e1.record()
         ctx.synchronize()
         ms = e0.elapsed_ms(e1)
+        tb.close()
         prof = None
         if profile:
+            nat.Profiler.reset()
+            nat.Profiler.enable(True)
+            sc.run([warmup])
+            ctx.synchronize()
             prof = nat.Profiler.read()
             nat.Profiler.enable(False)
         return ms, prof
@@ -406,7 +414,7 @@ def run_c2(args):
 
     steps32 = max(1, args.steps32)
     sc32 = build("fp32", np.complex64)
-    ms32, prof32 = timed(sc32, steps32, 0, True)
+    ms32, prof32 = timed(sc32, steps32, 1, True)
     res32 = sc32.slice_results(steps32 * n_batch)
     log(f"c2: fp32 {ms32 / steps32:.1f} ms/execution, "
         f"{exec_flops * steps32 / ms32 / 1e9:.1f} TFLOP/s")

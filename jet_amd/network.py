@@ -233,6 +233,16 @@ class SlicedContraction:
             else:
                 self.net.execute(ptrs, out)
 
+    def capture(self, loop_iterations: Sequence[int]):
+        """One hipGraph over the given loop iterations (jet_amd.batch.Batch):
+        a single launch then runs all of them, each writing its n_batch
+        per-slice results."""
+        from .batch import Batch
+        ex = [(self.net, self._leaf_ptrs(self.loop_values(i)),
+               self.results.ptr + i * self.n_batch * self.dtype.itemsize)
+              for i in loop_iterations]
+        return Batch(ex, self.ctx)
+
     def slice_results(self, count: Optional[int] = None) -> np.ndarray:
         n = self.n_slices if count is None else count
         host = np.empty(n, dtype=self.dtype)
