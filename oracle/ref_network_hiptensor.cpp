@@ -100,8 +100,9 @@ void CaseContract()
     EXPECT(nodes[3].contracted && !nodes[4].contracted);
     // PathInfo on the same network: flops |out| * 2 * prod(contracted)
     EXPECT(info.GetNumLeaves() == 3);
-    EXPECT(info.GetPathStepFlops(3) == 4.0 * 2.0 * 2.0);  // B1,D3 out; C2 contracted
-    EXPECT(info.GetPathStepFlops(4) == 4.0 * 2.0 * 3.0);  // A0,D3 out; B1 contracted
+    // step 3: B1 (3) x D3 (2) out, C2 (2) contracted; step 4: A0 x D3 out, B1 contracted
+    EXPECT(info.GetPathStepFlops(3) == (3.0 * 2.0) * 2.0 * 2.0);
+    EXPECT(info.GetPathStepFlops(4) == (2.0 * 2.0) * 2.0 * 3.0);
 }
 
 void CaseImplicit()
