@@ -648,7 +648,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--c3-slices", type=int, default=256,
                     help="c3: slices of the c2 amplitude in the batch")
-    ap.add_argument("--batch-log2", type=int, default=2,
+    ap.add_argument("--batch-log2", type=int, default=6,
                     help="c2: sliced indices batched per execution (2^b slices per step)")
     ap.add_argument("--split", default="f16", choices=sorted(SPLIT_MODES),
                     help="split 16-bit MFMA GEMM: f16 (split-FP16 4M), 4m (split-BF16 "

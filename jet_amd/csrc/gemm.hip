@@ -1007,7 +1007,7 @@ void launch_gemm_f16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                         hipStream_t stream)
 {
-    constexpr int BM = 128, BN = 128, WM = 2, WN = 4;  // 8 waves: two per SIMD
+    constexpr int BM = 128, BN = 128, WM = 4, WN = 4;  // 16 waves: four per SIMD
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
     JH_REQUIRE(a.planes == 4 && b.planes == 4 && a.interleaved && b.interleaved && a.scale &&
                    b.scale,
@@ -1029,7 +1029,7 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K
                          int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                          hipStream_t stream)
 {
-    constexpr int BM = 128, BN = 128, WM = 2, WN = 4;  // 8 waves: two per SIMD
+    constexpr int BM = 128, BN = 128, WM = 4, WN = 4;  // 16 waves: four per SIMD
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
     JH_REQUIRE(a.planes == 4 && b.planes == 4 && a.interleaved && b.interleaved,
                "The 4M kernel needs four interleaved planes.");
