@@ -363,20 +363,30 @@ def run_c2(args):
         warm-up executions as another, launched first).  With `profile`, one
         more execution runs eagerly under the event profiler afterwards: the
         per-kernel times of the roofline."""
-        if warmup:
-            wb = sc.capture(range(warmup))
-            wb.launch()
+        if args.eager:
+            sc.run(range(warmup))
             ctx.synchronize()
-            wb.close()
-        tb = sc.capture(range(warmup, warmup + iters))
-        ctx.synchronize()
-        e0, e1 = nat.Event(ctx), nat.Event(ctx)
-        e0.record()
-        tb.launch()
-        e1.record()
-        ctx.synchronize()
-        ms = e0.elapsed_ms(e1)
-        tb.close()
+            e0, e1 = nat.Event(ctx), nat.Event(ctx)
+            e0.record()
+            sc.run(range(warmup, warmup + iters))
+            e1.record()
+            ctx.synchronize()
+            ms = e0.elapsed_ms(e1)
+        else:
+            if warmup:
+                wb = sc.capture(range(warmup))
+                wb.launch()
+                ctx.synchronize()
+                wb.close()
+            tb = sc.capture(range(warmup, warmup + iters))
+            ctx.synchronize()
+            e0, e1 = nat.Event(ctx), nat.Event(ctx)
+            e0.record()
+            tb.launch()
+            e1.record()
+            ctx.synchronize()
+            ms = e0.elapsed_ms(e1)
+            tb.close()
         prof = None
         if profile:
             nat.Profiler.reset()
@@ -658,6 +668,9 @@ def main():
     ap.add_argument("--steps32", type=int, default=1,
                     help="c2: executions timed in plain FP32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="c2: launch the timed executions eagerly instead of as one "
+                         "captured hipGraph (for kernel tracers)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--quick", action="store_true",
                     help="only the timed split-BF16 region (for profiler runs)")
