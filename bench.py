@@ -441,6 +441,8 @@ def run_c2(args):
     err32 = float(np.linalg.norm(res32[:n_ref] - res64) / np.linalg.norm(res64))
     amp16 = complex(np.sum(res16[:n_ref].astype(np.complex128)))
     amp_err16 = abs(amp16 - sum64) / abs(sum64)
+    amp32 = complex(np.sum(res32[:n_ref].astype(np.complex128)))
+    amp_err32 = abs(amp32 - sum64) / abs(sum64)
     log(f"c2: fp64 {ms64:.1f} ms/execution; rel err per-slice {label} {err16:.2e} "
         f"fp32 {err32:.2e}; partial-amplitude err {amp_err16:.2e}")
     compare = {}
@@ -493,6 +495,7 @@ def run_c2(args):
         "projected_amplitude_s": {"split": round(4096 / sps16, 1), "fp32": round(4096 / sps32, 1)},
         "rel_err_vs_fp64": {"split_per_slice": err16, "fp32_per_slice": err32,
                             "split_partial_amplitude": amp_err16,
+                            "fp32_partial_amplitude": amp_err32,
                             "slices_compared": n_ref},
         "other_split_modes": compare,
         "fp64_mfma": {"ms_per_step": round(ms64, 3)},
@@ -658,7 +661,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--c3-slices", type=int, default=256,
                     help="c3: slices of the c2 amplitude in the batch")
-    ap.add_argument("--batch-log2", type=int, default=6,
+    ap.add_argument("--batch-log2", type=int, default=7,
                     help="c2: sliced indices batched per execution (2^b slices per step)")
     ap.add_argument("--split", default="f16", choices=sorted(SPLIT_MODES),
                     help="split 16-bit MFMA GEMM: f16 (split-FP16 4M), 4m (split-BF16 "
