@@ -265,5 +265,14 @@ class UnevenBatch:
         first = job.first_execution * sc.n_batch
         return sc.slice_results(first + job.executions * sc.n_batch)[first:]
 
+    def close(self) -> None:
+        """Frees the graph and every job's device memory now."""
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
+        for part in self.parts:
+            part.net.close()
+        self.parts = []
+
     def forms(self) -> List[str]:
         return [p.form if isinstance(p, _Amplitudes) else "slices" for p in self.parts]
