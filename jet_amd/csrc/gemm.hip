@@ -637,24 +637,33 @@ template <> struct Mfma16<_Float16> {
 
 // E = __bf16: split-BF16; E = _Float16: split-FP16, whose operand rows carry
 // power-of-two scales that the epilogue applies (a.scale / b.scale).
+//
+// Each k tile (32 wide) of a 128-row operand panel is 128 rows x 256 bytes
+// (4 planes x 32 values).  It goes from global memory straight into LDS with
+// global_load_lds_dwordx4 (no register staging): one instruction moves 64
+// 16-byte chunks to 64 consecutive LDS slots, so LDS row r holds its 16
+// chunks at slot (plane * 4 + chunk) ^ (r & 15).  That xor makes the 16 rows
+// read by one MFMA operand fetch (same plane and chunk) land in 16 different
+// bank groups, and lets each load instruction read 4 whole 256-byte rows.
+// Rows past the padded operand are read from its last row: they only feed
+// output rows or columns that are not stored.  Two LDS buffers (128 KiB):
+// the next k tile streams in while the current one is multiplied.
 template <class E, int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_kernel(
     SplitOperand a, SplitOperand b, float2 *__restrict__ Cout, int64_t M,
     int64_t N, int64_t ldc, int64_t sC, int a_bcast, int b_bcast)
 {
     using vec = typename Mfma16<E>::vec;
-    constexpr int NT = 64 * WM * WN;
+    typedef __attribute__((address_space(3))) void lds_void;
+    constexpr int NW = WM * WN;
     constexpr int KT = 32;
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
-    constexpr int A_CH = 4 * BM * (KT / 8) / NT; // 16-byte chunks per thread
-    constexpr int B_CH = 4 * BN * (KT / 8) / NT;
-    static_assert(A_CH * NT == 4 * BM * 4 && B_CH * NT == 4 * BN * 4, "tile");
-    // LDS rows of 32 16-bit values (four 16-byte chunks) without padding; chunk kc of
-    // row r is stored at kc ^ ((r >> 2) & 3), which spreads the 16 rows read
-    // by one MFMA operand fetch over all 64 banks.
-    __shared__ vec As[2][4 * BM * 4];
-    __shared__ vec Bs[2][4 * BN * 4];
+    constexpr int A_IN = BM * 16 / (64 * NW); // load instructions per wave per k tile
+    constexpr int B_IN = BN * 16 / (64 * NW);
+    static_assert(A_IN * 64 * NW == BM * 16 && B_IN * 64 * NW == BN * 16, "tile");
+    __shared__ vec As0[BM * 16], As1[BM * 16];
+    __shared__ vec Bs0[BN * 16], Bs1[BN * 16];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
@@ -663,10 +672,40 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_kernel(
     tile_coords(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
     const int64_t m0 = tm * BM, n0 = tn * BN;
     // interleaved layout: a batch holds rows_p * k_p * 4 values
-    const E *Ab = static_cast<const E *>(a.base) +
-                       (a_bcast ? 0 : bz * a.rows_p * a.k_p * 4);
-    const E *Bb = static_cast<const E *>(b.base) +
-                       (b_bcast ? 0 : bz * b.rows_p * b.k_p * 4);
+    const E *Ab = static_cast<const E *>(a.base) + (a_bcast ? 0 : bz * a.rows_p * a.k_p * 4);
+    const E *Bb = static_cast<const E *>(b.base) + (b_bcast ? 0 : bz * b.rows_p * b.k_p * 4);
+    const int64_t nkt_a = a.k_p / KT, nkt_b = b.k_p / KT;
+
+    // Source of this lane's chunks (elements, k tile 0).
+    int64_t offa[A_IN], offb[B_IN];
+#pragma unroll
+    for (int i = 0; i < A_IN; i++) {
+        const int p = (wave * A_IN + i) * 64 + lane;
+        const int r = p >> 4, slot = p & 15;
+        int64_t m = m0 + r;
+        if (m >= a.rows_p) m = a.rows_p - 1;
+        offa[i] = (m * nkt_a * 16 + (slot ^ (r & 15))) * 8;
+    }
+#pragma unroll
+    for (int i = 0; i < B_IN; i++) {
+        const int p = (wave * B_IN + i) * 64 + lane;
+        const int r = p >> 4, slot = p & 15;
+        int64_t n = n0 + r;
+        if (n >= b.rows_p) n = b.rows_p - 1;
+        offb[i] = (n * nkt_b * 16 + (slot ^ (r & 15))) * 8;
+    }
+    auto issue = [&](int kt, vec *as, vec *bs) {
+#pragma unroll
+        for (int i = 0; i < A_IN; i++)
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(Ab + offa[i] + int64_t(kt) * 128),
+                (lds_void *)(as + (wave * A_IN + i) * 64), 16, 0, 0);
+#pragma unroll
+        for (int i = 0; i < B_IN; i++)
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(Bb + offb[i] + int64_t(kt) * 128),
+                (lds_void *)(bs + (wave * B_IN + i) * 64), 16, 0, 0);
+    };
 
     f32x4 cr[FM][FN], ci[FM][FN];
 #pragma unroll
@@ -677,73 +716,24 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_kernel(
             ci[i][jj] = f32x4{0, 0, 0, 0};
         }
 
-    vec ra[A_CH], rb[B_CH];
-    const vec zero = {};
-    // Chunk c of a tile: row c / 16, then 16 chunks of 16 bytes covering the
-    // 256 contiguous bytes of that row's k tile (plane (c % 16) / 4, column
-    // chunk c % 4).
-    const int64_t nkt_a = a.k_p / KT, nkt_b = b.k_p / KT;
-    auto gload = [&](int kt) {
-#pragma unroll
-        for (int i = 0; i < A_CH; i++) {
-            const int c = tid + i * NT;
-            const int r = c / 16, w = c % 16;
-            const int64_t m = m0 + r;
-            ra[i] = (m < a.rows_p) ? *reinterpret_cast<const vec *>(
-                                         Ab + ((m * nkt_a + kt) * 16 + w) * 8)
-                                   : zero;
-        }
-#pragma unroll
-        for (int i = 0; i < B_CH; i++) {
-            const int c = tid + i * NT;
-            const int r = c / 16, w = c % 16;
-            const int64_t n = n0 + r;
-            rb[i] = (n < b.rows_p) ? *reinterpret_cast<const vec *>(
-                                         Bb + ((n * nkt_b + kt) * 16 + w) * 8)
-                                   : zero;
-        }
-    };
-    auto swrite = [&](int buf) {
-#pragma unroll
-        for (int i = 0; i < A_CH; i++) {
-            const int c = tid + i * NT;
-            const int r = c / 16, pl = (c % 16) / 4, kc = c % 4;
-            As[buf][(pl * BM + r) * 4 + (kc ^ ((r >> 2) & 3))] = ra[i];
-        }
-#pragma unroll
-        for (int i = 0; i < B_CH; i++) {
-            const int c = tid + i * NT;
-            const int r = c / 16, pl = (c % 16) / 4, kc = c % 4;
-            Bs[buf][(pl * BN + r) * 4 + (kc ^ ((r >> 2) & 3))] = rb[i];
-        }
-    };
-
-    const int nk = int(a.k_p / KT);
     const int q = lane >> 4, j = lane & 15;
-    gload(0);
-    swrite(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; kt++) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) gload(kt + 1);
+    auto compute = [&](const vec *as, const vec *bs) {
         vec brh[FN], brl[FN], bih[FN], bil[FN];
 #pragma unroll
         for (int fn = 0; fn < FN; fn++) {
             const int r = wn * WTN + fn * 16 + j;
-            const int sw = q ^ ((r >> 2) & 3);
-            brh[fn] = Bs[buf][(0 * BN + r) * 4 + sw];
-            brl[fn] = Bs[buf][(1 * BN + r) * 4 + sw];
-            bih[fn] = Bs[buf][(2 * BN + r) * 4 + sw];
-            bil[fn] = Bs[buf][(3 * BN + r) * 4 + sw];
+            brh[fn] = bs[r * 16 + ((0 * 4 + q) ^ j)];
+            brl[fn] = bs[r * 16 + ((1 * 4 + q) ^ j)];
+            bih[fn] = bs[r * 16 + ((2 * 4 + q) ^ j)];
+            bil[fn] = bs[r * 16 + ((3 * 4 + q) ^ j)];
         }
 #pragma unroll
         for (int fm = 0; fm < FM; fm++) {
             const int r = wm * WTM + fm * 16 + j;
-            const int sw = q ^ ((r >> 2) & 3);
-            const vec arh = As[buf][(0 * BM + r) * 4 + sw];
-            const vec arl = As[buf][(1 * BM + r) * 4 + sw];
-            const vec aih = As[buf][(2 * BM + r) * 4 + sw];
-            const vec ail = As[buf][(3 * BM + r) * 4 + sw];
+            const vec arh = as[r * 16 + ((0 * 4 + q) ^ j)];
+            const vec arl = as[r * 16 + ((1 * 4 + q) ^ j)];
+            const vec aih = as[r * 16 + ((2 * 4 + q) ^ j)];
+            const vec ail = as[r * 16 + ((3 * 4 + q) ^ j)];
             const vec naih = neg16x8(aih), nail = neg16x8(ail);
 #pragma unroll
             for (int fn = 0; fn < FN; fn++) {
@@ -764,7 +754,24 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_kernel(
                 ci[fm][fn] = y;
             }
         }
-        if (kt + 1 < nk) swrite(buf ^ 1);
+    };
+
+    // vmcnt(0) (expcnt and lgkmcnt left at their maximum): this wave's
+    // direct-to-LDS loads have landed; the barrier then covers all waves.
+    constexpr int kWaitLoads = 0x0f70;
+    const int nk = int(a.k_p / KT);
+    issue(0, As0, Bs0);
+    __builtin_amdgcn_s_waitcnt(kWaitLoads);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt += 2) {
+        if (kt + 1 < nk) issue(kt + 1, As1, Bs1);
+        compute(As0, Bs0);
+        __builtin_amdgcn_s_waitcnt(kWaitLoads);
+        __syncthreads();
+        if (kt + 1 >= nk) break;
+        if (kt + 2 < nk) issue(kt + 2, As0, Bs0);
+        compute(As1, Bs1);
+        __builtin_amdgcn_s_waitcnt(kWaitLoads);
         __syncthreads();
     }
 
@@ -1007,7 +1014,7 @@ void launch_gemm_f16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                         hipStream_t stream)
 {
-    constexpr int BM = 128, BN = 128, WM = 4, WN = 4;  // 16 waves: four per SIMD
+    constexpr int BM = 128, BN = 128, WM = 2, WN = 4;  // 8 waves: two per SIMD
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
     JH_REQUIRE(a.planes == 4 && b.planes == 4 && a.interleaved && b.interleaved && a.scale &&
                    b.scale,
@@ -1029,7 +1036,7 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K
                          int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                          hipStream_t stream)
 {
-    constexpr int BM = 128, BN = 128, WM = 4, WN = 4;  // 16 waves: four per SIMD
+    constexpr int BM = 128, BN = 128, WM = 2, WN = 4;  // 8 waves: two per SIMD
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
     JH_REQUIRE(a.planes == 4 && b.planes == 4 && a.interleaved && b.interleaved,
                "The 4M kernel needs four interleaved planes.");
