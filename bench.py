@@ -65,8 +65,8 @@ SPLIT_MODES = {"3m": ("PREC_BF16X3", "bf16x3", "gemm_bf16x3_mfma", 18 / 8),
 SPLIT_LABEL = {"3m": "split-BF16 3M", "4m": "split-BF16 4M", "f16": "split-FP16 4M"}
 
 # Kernel names in rocprofv3 summaries for the event profiler's kernel families.
-ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_bf16x4m_kernel"),
-                "gemm_f16x3_4m_mfma": ("cgemm_x4m_kernel f16",),
+ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_x4m_wide_kernel bf16"),
+                "gemm_f16x3_4m_mfma": ("cgemm_x4m_kernel f16", "cgemm_x4m_wide_kernel f16"),
                 "gemm_bf16x3_mfma": ("cgemm_bf16x3_kernel",),
                 "gemm_fp32_mfma": ("cgemm_embed_kernel f32",),
                 "gemm_fp64_mfma": ("cgemm_embed_kernel f64",),
@@ -74,24 +74,28 @@ ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_bf16x4m_
 
 
 def profiled_traffic(summary: str, family: str):
-    """HBM bytes per dispatch of a kernel, from the traffic table of a
-    committed rocprofv3 summary under profiles/ (FETCH_SIZE and WRITE_SIZE
-    passes, scaled by the copy-kernel calibration); None if absent."""
+    """HBM bytes per dispatch of a kernel family (dispatch-weighted over its
+    kernels), from the traffic table of a committed rocprofv3 summary under
+    profiles/ (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel
+    calibration); None if absent."""
     path = os.path.join(ROOT, "profiles", summary)
     names = ROCPROF_NAME.get(family)
     if names is None or not os.path.exists(path):
         return None
     in_traffic = False
+    total, dispatches = 0.0, 0
     for line in open(path):
         if line.startswith("## HBM traffic"):
             in_traffic = True
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if in_traffic and len(cells) >= 5 and cells[0] in names:
             try:
-                return float(cells[4]) * 1e9
+                n = int(cells[1])
+                total += n * float(cells[4]) * 1e9
+                dispatches += n
             except ValueError:
                 return None
-    return None
+    return total / dispatches if dispatches else None
 
 
 def log(msg):

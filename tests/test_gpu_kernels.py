@@ -162,6 +162,8 @@ def test_contract_random_small(ctx, prec, dtype):
     (nat.PREC_F16X3_4M, np.complex64, (512, 512, 512)),
     (nat.PREC_F16X3_4M, np.complex64, (300, 200, 170)),
     (nat.PREC_F16X3_4M, np.complex64, (1000, 260, 1100)),
+    (nat.PREC_F16X3_4M, np.complex64, (300, 4100, 200)),   # 128x192 tiles, ragged edges
+    (nat.PREC_BF16X3_4M, np.complex64, (130, 4500, 160)),
     (nat.PREC_FP64, np.complex128, (256, 128, 96)),
     (nat.PREC_FP64, np.complex128, (130, 70, 50)),
     (nat.PREC_FP32, np.complex64, (8, 4, 4000)),
