@@ -665,7 +665,8 @@ def run_c3(args):
 def run_c4(args):
     """Gradient of L = |<b|C(theta)|0>|^2 for a seeded 6x6 depth-10 grid
     circuit whose ~300 fSim angles are ~U(0, 2 pi): one forward contraction
-    plus the adjoint (environment) pass on the GPU through the
+    plus the adjoint (environment) pass on the GPU, planned as one
+    contraction DAG and replayed as a hipGraph, through the
     torch.autograd.Function; a step is one loss + gradient evaluation.
     Checked against central differences (h = 1e-3, complex128 on FP64 MFMA)
     on --c4-fd angles (all of them by default)."""
@@ -677,7 +678,7 @@ def run_c4(args):
     net = circuits.amplitude_network(circ)
     dims = {l: 2 for ix in net.indices for l in ix}
     path = paths.best_minfill_path(net.indices, dims, trials=8, seed=1)
-    prob = ParameterizedAmplitude(circ, path, checkpoint_every=4)
+    prob = ParameterizedAmplitude(circ, path)
     theta = torch.tensor(prob.theta, dtype=torch.float64, requires_grad=True)
     ctx = prob.ctx
     for _ in range(args.warmup):
@@ -715,7 +716,8 @@ def run_c4(args):
                    "angles": prob.n_params, "path_steps": len(path)},
         "rel_err_vs_central_differences": err,
         "angles_checked": n_fd,
-        "recomputed_steps": prob.adjoint.recomputed_steps,
+        "method": prob.method,
+        "dag_steps": len(prob.program.steps) if prob.method == "graph" else None,
     }, None
 
 

@@ -242,6 +242,36 @@ int jet_hip_network_capture(jet_hip_network_t net, const void *const *leaves,
 int jet_hip_network_launch(jet_hip_network_t net, void *stream);
 int jet_hip_network_destroy(jet_hip_network_t net);
 
+/* ---- contraction DAGs: sequences whose intermediates feed several steps
+ *      (a forward contraction and its adjoint, environment pass).  The GPU
+ *      counterpart of a TaskBasedContractor task graph in which tasks share
+ *      subtrees by name (include/jet/TaskBasedContractor.hpp:206-240).
+ * Step s contracts nodes steps[2s] and steps[2s+1] (leaves 0..n_leaves-1,
+ * step s produces node n_leaves+s) over all the modes they share; its
+ * result holds the first operand's other modes, then the second's.  A node
+ * may feed any number of later steps.  The results of the steps listed in
+ * keep (node ids) stay in the arena after an execution, in one contiguous
+ * region in keep order; jet_hip_dag_value gives their addresses.  Executed
+ * with jet_hip_network_execute / _capture / _launch with out = NULL, and
+ * destroyed with jet_hip_network_destroy.                                  */
+int jet_hip_dag_create(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
+                       jet_hip_precision_t precision, int n_leaves,
+                       const int32_t *leaf_ranks, const int32_t *leaf_modes,
+                       const int64_t *leaf_extents, int n_steps,
+                       const int32_t *steps, int n_keep, const int32_t *keep,
+                       jet_hip_network_t *net);
+/* The plan of jet_hip_dag_create without a device or any allocation. */
+int jet_hip_dag_estimate(jet_hip_dtype_t dtype, jet_hip_precision_t precision,
+                         int n_leaves, const int32_t *leaf_ranks,
+                         const int32_t *leaf_modes, const int64_t *leaf_extents,
+                         int n_steps, const int32_t *steps, int n_keep,
+                         const int32_t *keep, double *flops,
+                         size_t *arena_bytes, size_t *workspace_bytes);
+/* Device address, rank and modes / extents (arrays of at least 64 entries,
+ * may be NULL) of a kept step result. */
+int jet_hip_dag_value(jet_hip_network_t net, int node, void **ptr, int *rank,
+                      int32_t *modes, int64_t *extents);
+
 /* ---- uneven batches: executions of several planned networks (different
  *      circuits, bitstrings, slice ranges) captured as one hipGraph and
  *      launched at once.  Replaces the composition of many contraction

@@ -43,7 +43,8 @@ EXPORTS = [
     "jet_hip_profiler_kernel_name", "jet_hip_network_create", "jet_hip_network_estimate", "jet_hip_network_info",
     "jet_hip_network_step_info", "jet_hip_network_execute", "jet_hip_network_capture",
     "jet_hip_network_launch", "jet_hip_network_destroy", "jet_hip_batch_capture",
-    "jet_hip_batch_launch", "jet_hip_batch_destroy",
+    "jet_hip_batch_launch", "jet_hip_batch_destroy", "jet_hip_dag_create", "jet_hip_dag_estimate",
+    "jet_hip_dag_value",
 ]
 
 
@@ -120,6 +121,16 @@ def _declare(L):
                                   ctypes.POINTER(ctypes.POINTER(_vp)), ctypes.POINTER(_vp), _vp,
                                   ctypes.POINTER(_vp)],
         "jet_hip_batch_launch": [_vp, _vp],
+        "jet_hip_dag_create": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
+                               _i64p, ctypes.c_int, _i32p, ctypes.c_int, _i32p,
+                               ctypes.POINTER(_vp)],
+        "jet_hip_dag_estimate": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
+                                 _i64p, ctypes.c_int, _i32p, ctypes.c_int, _i32p,
+                                 ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_size_t),
+                                 ctypes.POINTER(ctypes.c_size_t)],
+        "jet_hip_dag_value": [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int),
+                              _i32p, _i64p],
         "jet_hip_batch_destroy": [_vp],
         "jet_hip_profiler_enable": [ctypes.c_int],
         "jet_hip_profiler_reset": [],

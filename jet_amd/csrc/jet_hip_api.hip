@@ -661,6 +661,78 @@ int jet_hip_network_estimate(jet_hip_dtype_t dtype, jet_hip_precision_t precisio
     });
 }
 
+int jet_hip_dag_create(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, jet_hip_precision_t precision,
+                       int n_leaves, const int32_t *leaf_ranks, const int32_t *leaf_modes,
+                       const int64_t *leaf_extents, int n_steps, const int32_t *steps,
+                       int n_keep, const int32_t *keep, jet_hip_network_t *net)
+{
+    return guarded([&] {
+        JH_REQUIRE(ctx && net && leaf_ranks && steps && (keep || n_keep == 0), "Null argument.");
+        set_device(ctx);
+        auto *n = new jet_hip_network_s;
+        try {
+            n->ctx = ctx;
+            n->plan = plan_dag(elem_bytes_of(dtype), mode_of(dtype, precision), n_leaves,
+                               leaf_ranks, leaf_modes, leaf_extents, n_steps, steps, n_keep,
+                               keep);
+            if (n->plan.arena_bytes) {
+                void *p = nullptr;
+                JH_HIP(hipMalloc(&p, n->plan.arena_bytes));
+                n->arena = static_cast<char *>(p);
+            }
+            if (n->plan.workspace_bytes) {
+                void *p = nullptr;
+                JH_HIP(hipMalloc(&p, n->plan.workspace_bytes));
+                n->workspace = static_cast<char *>(p);
+            }
+        }
+        catch (...) {
+            if (n->arena) (void)hipFree(n->arena);
+            if (n->workspace) (void)hipFree(n->workspace);
+            delete n;
+            throw;
+        }
+        *net = n;
+    });
+}
+
+int jet_hip_dag_estimate(jet_hip_dtype_t dtype, jet_hip_precision_t precision, int n_leaves,
+                         const int32_t *leaf_ranks, const int32_t *leaf_modes,
+                         const int64_t *leaf_extents, int n_steps, const int32_t *steps,
+                         int n_keep, const int32_t *keep, double *flops, size_t *arena_bytes,
+                         size_t *workspace_bytes)
+{
+    return guarded([&] {
+        JH_REQUIRE(leaf_ranks && steps && (keep || n_keep == 0), "Null argument.");
+        NetworkPlan np = plan_dag(elem_bytes_of(dtype), mode_of(dtype, precision), n_leaves,
+                                  leaf_ranks, leaf_modes, leaf_extents, n_steps, steps, n_keep,
+                                  keep);
+        if (flops) *flops = np.flops;
+        if (arena_bytes) *arena_bytes = np.arena_bytes;
+        if (workspace_bytes) *workspace_bytes = np.workspace_bytes;
+    });
+}
+
+int jet_hip_dag_value(jet_hip_network_t net, int node, void **ptr, int *rank, int32_t *modes,
+                      int64_t *extents)
+{
+    return guarded([&] {
+        JH_REQUIRE(net, "Null network.");
+        const NetworkPlan &np = net->plan;
+        JH_REQUIRE(np.dag, "Not a contraction DAG.");
+        const int s = node - np.n_leaves;
+        JH_REQUIRE(s >= 0 && s < (int)np.steps.size() && np.kept[s],
+                   "The node is not a kept step result.");
+        const NetworkStep &st = np.steps[s];
+        if (ptr) *ptr = net->arena + st.offset;
+        if (rank) *rank = (int)st.modes.size();
+        for (size_t i = 0; i < st.modes.size(); i++) {
+            if (modes) modes[i] = st.modes[i];
+            if (extents) extents[i] = st.extents[i];
+        }
+    });
+}
+
 int jet_hip_network_info(jet_hip_network_t net, double *flops, size_t *arena_bytes,
                          size_t *workspace_bytes, int64_t *out_size, int *n_steps)
 {
@@ -697,7 +769,7 @@ int jet_hip_network_execute(jet_hip_network_t net, const void *const *leaves, vo
                             void *stream)
 {
     return guarded([&] {
-        JH_REQUIRE(net && leaves && out, "Null argument.");
+        JH_REQUIRE(net && leaves && (out || net->plan.dag), "Null argument.");
         set_device(net->ctx);
         run_network(net, leaves, out, pick_stream(net->ctx, stream));
     });
@@ -707,7 +779,7 @@ int jet_hip_network_capture(jet_hip_network_t net, const void *const *leaves, vo
                             void *stream)
 {
     return guarded([&] {
-        JH_REQUIRE(net && leaves && out, "Null argument.");
+        JH_REQUIRE(net && leaves && (out || net->plan.dag), "Null argument.");
         set_device(net->ctx);
         hipStream_t s = pick_stream(net->ctx, stream);
         free_graph(net);
@@ -746,7 +818,8 @@ int jet_hip_batch_capture(jet_hip_ctx_t ctx, int n, const jet_hip_network_t *net
     return guarded([&] {
         JH_REQUIRE(ctx && batch && n >= 1 && networks && leaves && outs, "Null argument.");
         for (int i = 0; i < n; i++) {
-            JH_REQUIRE(networks[i] && leaves[i] && outs[i], "Null network, leaves or output.");
+            JH_REQUIRE(networks[i] && leaves[i] && (outs[i] || networks[i]->plan.dag),
+                       "Null network, leaves or output.");
             JH_REQUIRE(networks[i]->ctx == ctx, "Every network of a batch needs the batch's context.");
         }
         set_device(ctx);

@@ -212,4 +212,91 @@ NetworkPlan plan_network(int elem_bytes, int mode, int n_leaves,
     return np;
 }
 
+
+NetworkPlan plan_dag(int elem_bytes, int mode, int n_leaves, const int32_t *leaf_ranks,
+                     const int32_t *leaf_modes, const int64_t *leaf_extents, int n_steps,
+                     const int32_t *steps, int n_keep, const int32_t *keep)
+{
+    JH_REQUIRE(n_leaves >= 1 && n_steps >= 1 && n_keep >= 0, "Invalid contraction DAG.");
+    NetworkPlan np;
+    np.dag = true;
+    np.elem_bytes = elem_bytes;
+    np.n_leaves = n_leaves;
+    const int n_nodes = n_leaves + n_steps;
+    std::vector<std::vector<int32_t>> modes(n_nodes);
+    std::vector<std::vector<int64_t>> exts(n_nodes);
+    std::unordered_map<int32_t, int64_t> extent_of;
+    size_t pos = 0;
+    for (int i = 0; i < n_leaves; i++) {
+        JH_REQUIRE(leaf_ranks[i] >= 0 && leaf_ranks[i] <= kMaxRank, "Leaf rank must be 0..64.");
+        modes[i].assign(leaf_modes + pos, leaf_modes + pos + leaf_ranks[i]);
+        exts[i].assign(leaf_extents + pos, leaf_extents + pos + leaf_ranks[i]);
+        pos += leaf_ranks[i];
+        for (size_t j = 0; j < modes[i].size(); j++) {
+            auto it = extent_of.find(modes[i][j]);
+            JH_REQUIRE(it == extent_of.end() || it->second == exts[i][j],
+                       "An index has different extents in different tensors.");
+            extent_of[modes[i][j]] = exts[i][j];
+        }
+    }
+    np.leaf_modes.assign(modes.begin(), modes.begin() + n_leaves);
+    np.leaf_extents.assign(exts.begin(), exts.begin() + n_leaves);
+
+    std::vector<int> last_use(n_nodes, -1);
+    for (int s = 0; s < n_steps; s++) {
+        const int a = steps[2 * s], b = steps[2 * s + 1];
+        JH_REQUIRE(a >= 0 && b >= 0 && a < n_leaves + s && b < n_leaves + s && a != b,
+                   "Node ID in contraction path pair is invalid.");
+        last_use[a] = last_use[b] = s;
+    }
+    np.kept.assign(n_steps, 0);
+    std::vector<int> keep_order;
+    for (int i = 0; i < n_keep; i++) {
+        const int id = keep[i];
+        JH_REQUIRE(id >= n_leaves && id < n_nodes, "Only step results can be kept.");
+        JH_REQUIRE(!np.kept[id - n_leaves], "A step result is kept twice.");
+        np.kept[id - n_leaves] = 1;
+        keep_order.push_back(id - n_leaves);
+    }
+
+    ArenaSim arena;
+    std::vector<size_t> offset(n_nodes, 0);
+    np.steps.resize(n_steps);
+    for (int s = 0; s < n_steps; s++) {
+        NetworkStep &st = np.steps[s];
+        st.a = steps[2 * s];
+        st.b = steps[2 * s + 1];
+        const auto &A = modes[st.a], &B = modes[st.b];
+        for (int32_t m : A)
+            if (find(B, m) < 0) st.modes.push_back(m);
+        for (int32_t m : B)
+            if (find(A, m) < 0) st.modes.push_back(m);
+        for (int32_t m : st.modes) st.extents.push_back(extent_of[m]);
+        modes[n_leaves + s] = st.modes;
+        exts[n_leaves + s] = st.extents;
+        st.desc = plan_contraction(elem_bytes, mode, 0, (int)A.size(), A.data(),
+                                   exts[st.a].data(), (int)B.size(), B.data(),
+                                   exts[st.b].data(), (int)st.modes.size(), st.modes.data());
+        np.flops += st.desc.flops;
+        np.workspace_bytes = std::max(np.workspace_bytes, st.desc.workspace);
+        if (!np.kept[s]) {
+            offset[n_leaves + s] = arena.alloc((size_t)elem_bytes * (size_t)st.desc.size_c);
+            st.offset = offset[n_leaves + s];
+        }
+        for (int id : {st.a, st.b})
+            if (id >= n_leaves && last_use[id] == s && !np.kept[id - n_leaves])
+                arena.release(offset[id]);
+        // A result nothing uses is released at once (unless kept).
+        if (last_use[n_leaves + s] < 0 && !np.kept[s]) arena.release(st.offset);
+    }
+    // Kept results: one region after the working arena, in `keep` order.
+    size_t top = align256(arena.high_water());
+    for (int s : keep_order) {
+        np.steps[s].offset = top;
+        top += align256(std::max<size_t>(256, (size_t)elem_bytes * (size_t)np.steps[s].desc.size_c));
+    }
+    np.arena_bytes = top;
+    return np;
+}
+
 } // namespace jet_hip

@@ -48,6 +48,11 @@ struct NetworkPlan {
     size_t workspace_bytes = 0;
     double flops = 0;                 // sum of 8*M*N*K*batch over steps
     PermutePlan root_copy;            // used when there are no steps
+    // Contraction DAGs (plan_dag): no root and no output; `kept` marks the
+    // step results that stay in the arena (a contiguous region at its end,
+    // in the order they were listed) after an execution.
+    bool dag = false;
+    std::vector<char> kept;           // per step
 };
 
 NetworkPlan plan_network(int elem_bytes, int mode, int n_leaves,
@@ -55,5 +60,15 @@ NetworkPlan plan_network(int elem_bytes, int mode, int n_leaves,
                          const int64_t *leaf_extents, int n_steps,
                          const int32_t *path, int n_out,
                          const int32_t *out_modes);
+
+// A contraction DAG: step s contracts nodes steps[2s] and steps[2s+1]
+// (leaves 0..n_leaves-1; step s's result is node n_leaves + s) over all the
+// modes they share; the result holds the first operand's other modes, then
+// the second's.  A node may be an operand of several steps.  Results of the
+// steps in `keep` stay in the arena after the execution; the others are
+// released after their last use.
+NetworkPlan plan_dag(int elem_bytes, int mode, int n_leaves, const int32_t *leaf_ranks,
+                     const int32_t *leaf_modes, const int64_t *leaf_extents, int n_steps,
+                     const int32_t *steps, int n_keep, const int32_t *keep);
 
 } // namespace jet_hip

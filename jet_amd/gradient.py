@@ -15,6 +15,14 @@ intermediate (by step) and drops the rest once used; the reverse pass
 recomputes a dropped intermediate from its nearest kept descendants when it
 needs it.
 
+``AdjointProgram`` plans the same forward and reverse steps as one
+contraction DAG (``jet_hip_dag_*``) replayed as a hipGraph: every
+intermediate stays in the device arena until its last use (no
+recomputation), the environments that are asked for end in one contiguous
+region read back with a single copy.  It trades memory for launch overhead:
+the per-step host work of ``AdjointPath`` (planning lookups, pool
+allocations, one launch per kernel) is paid once, at capture.
+
 ``ParameterizedAmplitude`` applies this to the fSim gates of a grid circuit:
 dA/dtheta_k = sum(E_g * dG_g/dtheta_k), and the loss L = |A|^2 has
 dL/dtheta_k = 2 Re(conj(A) dA/dtheta_k).  ``AmplitudeLoss`` is the
@@ -30,7 +38,7 @@ import numpy as np
 from . import _native as nat
 from .abort import JetError
 from .circuits import Circuit, amplitude_network, fsim
-from .network import DeviceNetwork
+from .network import DeviceDag, DeviceNetwork
 from .tensor import Tensor
 from .utilities import vector_disjunctive_union
 
@@ -136,12 +144,101 @@ class AdjointPath:
         return out
 
 
+class AdjointProgram:
+    """The root value and the environments of ``wanted`` leaves of a closed
+    network along an SSA path, as one planned contraction DAG on the GPU.
+
+    Leaves are the network's leaves plus one scalar 1 (the root's
+    environment).  Steps: the forward path, then for each forward step
+    z = (x, y) taken in reverse whose subtree holds a wanted leaf,
+    E_x = contract(E_z, y) and E_y = contract(E_z, x) where needed (the
+    reverse pass of ``AdjointPath.environments``)."""
+
+    def __init__(self, indices: Sequence[Sequence[str]], shapes, path: Sequence[Sequence[int]],
+                 wanted: Sequence[int], dtype=np.complex64, precision: Optional[str] = None,
+                 ctx=None):
+        counts: Dict[str, int] = {}
+        for ix in indices:
+            for l in ix:
+                counts[l] = counts.get(l, 0) + 1
+        if any(c != 2 for c in counts.values()):
+            raise JetError("Environments need a closed network: every index on two tensors.")
+        n = len(indices)
+        path = [(int(a), int(b)) for (a, b) in path]
+        if len(path) != n - 1:
+            raise JetError("The path must contract the network to one tensor.")
+        self.n = n
+        self.one = n                     # the scalar leaf
+        n_dag_leaves = n + 1
+        steps: List[tuple] = []
+        dag_id: Dict[int, int] = {i: i for i in range(n)}   # forward node -> DAG node
+        for s, (a, b) in enumerate(path):
+            steps.append((dag_id[a], dag_id[b]))
+            dag_id[n + s] = n_dag_leaves + len(steps) - 1
+        root = n + len(path) - 1
+        self.wanted = [int(w) for w in wanted]
+        need: Set[int] = set(self.wanted)
+        for s, (a, b) in enumerate(path):
+            if a in need or b in need:
+                need.add(n + s)
+        env: Dict[int, int] = {root: self.one}
+        self.env_node: Dict[int, int] = {}
+        for s in range(len(path) - 1, -1, -1):
+            ez = env.pop(n + s, None)
+            if ez is None:
+                continue
+            x, y = path[s]
+            for (u, other) in ((x, y), (y, x)):
+                if u not in need:
+                    continue
+                steps.append((ez, dag_id[other]))
+                e = n_dag_leaves + len(steps) - 1
+                if u < n:
+                    self.env_node[u] = e
+                else:
+                    env[u] = e
+        missing = [w for w in self.wanted if w not in self.env_node]
+        if missing:
+            raise JetError("No environment for leaves %s." % missing)
+        self.root_node = dag_id[root]
+        self.steps = steps
+        self.leaf_labels = [list(ix) for ix in indices]
+        self.dag = DeviceDag(self.leaf_labels + [[]], [list(sh) for sh in shapes] + [[]], steps,
+                             [self.root_node] + [self.env_node[w] for w in self.wanted],
+                             dtype=dtype, precision=precision, ctx=ctx)
+        self.ctx = self.dag.ctx
+        self.dtype = self.dag.dtype
+        self._one = nat.DeviceBuffer(16, self.ctx)
+        self._one.upload(np.ones(1, self.dtype))
+
+    def run(self, leaf_ptrs: Sequence[int]):
+        """(root value, {leaf: environment in the leaf's label order})."""
+        if len(leaf_ptrs) != self.n:
+            raise JetError("Wrong number of leaf tensors.")
+        self.dag.launch(list(leaf_ptrs) + [self._one.ptr])
+        vals = self.dag.kept_values()
+        root = complex(vals[self.root_node][1].reshape(-1)[0])
+        envs = {}
+        for w in self.wanted:
+            labels, data = vals[self.env_node[w]]
+            want = self.leaf_labels[w]
+            envs[w] = np.transpose(data, [labels.index(l) for l in want]) if want else data
+        return root, envs
+
+
 class ParameterizedAmplitude:
     """<b| C(theta) |0> of a grid circuit whose fSim gates carry their own
     angles (theta_k, phi_k), with the GPU adjoint gradient of |A|^2."""
 
     def __init__(self, circ: Circuit, path: Sequence[Sequence[int]], dtype=np.complex64,
-                 precision: Optional[str] = None, checkpoint_every: int = 4, ctx=None):
+                 precision: Optional[str] = None, checkpoint_every: int = 4, ctx=None,
+                 method: str = "graph"):
+        """method "graph": ``AdjointProgram`` (one captured DAG, every
+        intermediate kept until its last use); "eager": ``AdjointPath``
+        (per-step calls, checkpointed intermediates recomputed)."""
+        if method not in ("graph", "eager"):
+            raise JetError(f"Unknown gradient method '{method}'.")
+        self.method = method
         self.circ = circ
         self.ctx = ctx or nat.default_context()
         self.dtype = np.dtype(dtype)
@@ -162,17 +259,49 @@ class ParameterizedAmplitude:
         for gi, g in enumerate(circ.gates):
             if g.param_index is not None:
                 self.theta[g.param_index:g.param_index + 2] = g.params
-        self.leaves = [Tensor(ix, sh, d.astype(self.dtype), dtype=self.dtype, ctx=self.ctx)
-                       for ix, sh, d in zip(net.indices, net.shapes, net.data)]
-        self.adjoint = AdjointPath(net.indices, self.path, self.param_leaf,
-                                   checkpoint_every=checkpoint_every, precision=precision)
         self.amplitude: Optional[complex] = None
         self._fp64 = None
+        self._envs: Optional[Dict[int, np.ndarray]] = None
+        if method == "eager":
+            self.leaves = [Tensor(ix, sh, d.astype(self.dtype), dtype=self.dtype, ctx=self.ctx)
+                           for ix, sh, d in zip(net.indices, net.shapes, net.data)]
+            self.adjoint = AdjointPath(net.indices, self.path, self.param_leaf,
+                                       checkpoint_every=checkpoint_every, precision=precision)
+            return
+        self.adjoint = None
+        self.program = AdjointProgram(net.indices, net.shapes, self.path, self.param_leaf,
+                                      dtype=self.dtype, precision=precision, ctx=self.ctx)
+        # Every fSim gate in one device buffer (one upload per angle update).
+        item = self.dtype.itemsize
+        is_param = set(self.param_leaf)
+        self._bufs = []
+        self._leaf_ptrs = []
+        self._param_buf = nat.DeviceBuffer(len(self.param_leaf) * 16 * item, self.ctx)
+        pos = {leaf: j for j, leaf in enumerate(self.param_leaf)}
+        for i, d in enumerate(net.data):
+            if i in is_param:
+                self._leaf_ptrs.append(self._param_buf.ptr + pos[i] * 16 * item)
+            else:
+                b = nat.DeviceBuffer(max(1, d.size) * item, self.ctx)
+                b.upload(np.ascontiguousarray(d.reshape(-1), dtype=self.dtype))
+                self._bufs.append(b)
+                self._leaf_ptrs.append(b.ptr)
+        self._upload_params()
+
+    def _upload_params(self) -> None:
+        gates = np.stack([fsim(self.theta[slot], self.theta[slot + 1]).reshape(-1)
+                          for slot in self.param_slot]).astype(self.dtype)
+        self._param_buf.upload(np.ascontiguousarray(gates.reshape(-1)))
 
     def set_angles(self, theta: Sequence[float]) -> None:
         theta = np.asarray(theta, dtype=np.float64).reshape(-1)
         if theta.size != self.n_params:
             raise JetError("Wrong number of angles.")
+        if self.method == "graph":
+            if not np.array_equal(theta, self.theta):
+                self.theta = theta.copy()
+                self._upload_params()
+            return
         for leaf, slot in zip(self.param_leaf, self.param_slot):
             if theta[slot] != self.theta[slot] or theta[slot + 1] != self.theta[slot + 1]:
                 self.leaves[leaf].set_data(fsim(theta[slot], theta[slot + 1])
@@ -182,6 +311,9 @@ class ParameterizedAmplitude:
     def forward(self, theta: Optional[Sequence[float]] = None) -> complex:
         if theta is not None:
             self.set_angles(theta)
+        if self.method == "graph":
+            self.amplitude, self._envs = self.program.run(self._leaf_ptrs)
+            return self.amplitude
         root = self.adjoint.forward(self.leaves)
         self.amplitude = complex(root.get_data().reshape(-1)[0])
         return self.amplitude
@@ -189,17 +321,24 @@ class ParameterizedAmplitude:
     def amplitude_gradient(self) -> np.ndarray:
         """dA/dtheta (complex, one per angle) from the environments of the
         fSim gates; needs forward() first."""
-        envs = self.adjoint.environments()
         k = len(self.param_leaf)
-        buf = nat.DeviceBuffer(k * 16 * self.dtype.itemsize, self.ctx)
-        L = nat.lib()
-        for j, leaf in enumerate(self.param_leaf):
-            e = envs[leaf]
-            nat.check(L.jet_hip_memcpy(self.ctx.handle, buf.ptr + j * 16 * self.dtype.itemsize,
-                                       e.data_ptr, 16 * self.dtype.itemsize, 2, None, 1))
-        host = np.empty(k * 16, self.dtype)
-        buf.download(host)
-        E = host.astype(np.complex128).reshape(k, 16)
+        if self.method == "graph":
+            if self._envs is None:
+                raise JetError("Run forward() before amplitude_gradient().")
+            E = np.stack([self._envs[leaf].reshape(-1) for leaf in self.param_leaf]
+                         ).astype(np.complex128)
+        else:
+            envs = self.adjoint.environments()
+            buf = nat.DeviceBuffer(k * 16 * self.dtype.itemsize, self.ctx)
+            L = nat.lib()
+            for j, leaf in enumerate(self.param_leaf):
+                e = envs[leaf]
+                nat.check(L.jet_hip_memcpy(self.ctx.handle,
+                                           buf.ptr + j * 16 * self.dtype.itemsize,
+                                           e.data_ptr, 16 * self.dtype.itemsize, 2, None, 1))
+            host = np.empty(k * 16, self.dtype)
+            buf.download(host)
+            E = host.astype(np.complex128).reshape(k, 16)
         grad = np.zeros(self.n_params, dtype=np.complex128)
         for j, slot in enumerate(self.param_slot):
             th, ph = self.theta[slot], self.theta[slot + 1]

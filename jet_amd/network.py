@@ -143,6 +143,112 @@ class DeviceNetwork:
             pass
 
 
+class DeviceDag:
+    """A planned contraction DAG (``jet_hip_dag_*``): steps whose results may
+    feed several later steps, e.g. a forward contraction and its adjoint
+    pass.  Step s contracts nodes ``steps[s]`` (leaves 0..n-1, step s gives
+    node n + s) over all their shared labels; its result holds the first
+    operand's other labels, then the second's.  The results of ``keep`` stay
+    in one contiguous device region after an execution."""
+
+    def __init__(self, indices: Sequence[Sequence[str]], shapes, steps, keep: Sequence[int],
+                 dtype=np.complex64, precision: Optional[str] = None, ctx=None):
+        self.ctx = ctx or nat.default_context()
+        self.dtype = np.dtype(dtype)
+        code = nat.C64 if self.dtype == np.complex64 else nat.C128
+        self.labels = LabelIds()
+        ranks, modes, exts = _flatten(indices, shapes, self.labels)
+        flat = [int(x) for p in steps for x in p]
+        self.keep = [int(k) for k in keep]
+        self.n_leaves = len(indices)
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().jet_hip_dag_create(
+            self.ctx.handle, code, _PREC[precision], len(indices), nat.i32(ranks), nat.i32(modes),
+            nat.i64(exts), len(steps), nat.i32(flat), len(self.keep), nat.i32(self.keep),
+            ctypes.byref(h)))
+        self.handle = h
+        fl = ctypes.c_double()
+        ar, ws = ctypes.c_size_t(), ctypes.c_size_t()
+        nat.check(nat.lib().jet_hip_network_info(h, ctypes.byref(fl), ctypes.byref(ar),
+                                                 ctypes.byref(ws), None, None))
+        self.flops, self.arena_bytes, self.workspace_bytes = fl.value, ar.value, ws.value
+        names = {v: k for k, v in self.labels.ids.items()}
+        # kept values: (device pointer, labels, shape)
+        self.values = {}
+        for k in self.keep:
+            ptr, rank = ctypes.c_void_p(), ctypes.c_int()
+            m = (ctypes.c_int32 * 64)()
+            e = (ctypes.c_int64 * 64)()
+            nat.check(nat.lib().jet_hip_dag_value(h, k, ctypes.byref(ptr), ctypes.byref(rank),
+                                                  m, e))
+            r = rank.value
+            self.values[k] = (int(ptr.value or 0), [names[m[i]] for i in range(r)],
+                              [int(e[i]) for i in range(r)])
+        first = min(v[0] for v in self.values.values()) if self.values else 0
+        last = max(v[0] + max(1, int(np.prod(v[2]))) * self.dtype.itemsize
+                   for v in self.values.values()) if self.values else 0
+        self._region = (first, last - first)
+        self._graph_key = None
+
+    def _ptrs(self, leaves):
+        assert len(leaves) == self.n_leaves
+        return (ctypes.c_void_p * self.n_leaves)(*[int(x) for x in leaves])
+
+    def execute(self, leaf_ptrs: Sequence[int], stream=None) -> None:
+        nat.check(nat.lib().jet_hip_network_execute(self.handle, self._ptrs(leaf_ptrs), None,
+                                                    stream))
+
+    def launch(self, leaf_ptrs: Sequence[int], stream=None) -> None:
+        """Replay as a hipGraph (captured on first use for these pointers)."""
+        key = tuple(int(x) for x in leaf_ptrs)
+        if key != self._graph_key:
+            nat.check(nat.lib().jet_hip_network_capture(self.handle, self._ptrs(leaf_ptrs), None,
+                                                        stream))
+            self._graph_key = key
+        nat.check(nat.lib().jet_hip_network_launch(self.handle, stream))
+
+    def kept_values(self) -> Dict[int, Tuple[List[str], np.ndarray]]:
+        """{node: (labels, row-major data)} of the kept results, read back in
+        one copy of their region."""
+        first, size = self._region
+        raw = np.empty(size // self.dtype.itemsize, self.dtype)
+        nat.check(nat.lib().jet_hip_memcpy(self.ctx.handle, raw.ctypes.data, first, size, 1,
+                                           None, 0))
+        out = {}
+        for k, (ptr, labels, shape) in self.values.items():
+            o = (ptr - first) // self.dtype.itemsize
+            n = max(1, int(np.prod(shape)))
+            out[k] = (labels, raw[o:o + n].reshape(shape) if shape else raw[o:o + 1].reshape(()))
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            nat.check(nat.lib().jet_hip_network_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dag_estimate(indices, shapes, steps, keep, dtype=np.complex64,
+                 precision=None) -> Tuple[float, int, int]:
+    """(flops, arena bytes, workspace bytes) of a contraction DAG; no device."""
+    lab = LabelIds()
+    ranks, modes, exts = _flatten(indices, shapes, lab)
+    flat = [int(x) for p in steps for x in p]
+    fl = ctypes.c_double()
+    ar, ws = ctypes.c_size_t(), ctypes.c_size_t()
+    code = nat.C64 if np.dtype(dtype) == np.complex64 else nat.C128
+    nat.check(nat.lib().jet_hip_dag_estimate(
+        code, _PREC[precision], len(indices), nat.i32(ranks), nat.i32(modes), nat.i64(exts),
+        len(steps), nat.i32(flat), len(keep), nat.i32([int(k) for k in keep]),
+        ctypes.byref(fl), ctypes.byref(ar), ctypes.byref(ws)))
+    return fl.value, ar.value, ws.value
+
+
 def _slice_leaf(labels, shape, data, fixed: Dict[str, int]):
     arr = np.asarray(data).reshape(shape) if shape else np.asarray(data).reshape(())
     keep_l, keep_s, idx = [], [], []
