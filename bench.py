@@ -357,7 +357,7 @@ def run_c2(args):
         f"{len(path)} steps, per-slice PathInfo flops {plan['per_slice_pathinfo_flops']:.3e}, "
         f"batch {len(batch)} of {len(sliced)} sliced indices")
 
-    def build(prec, dtype):
+    def build(prec, dtype, batch=batch):
         data = [d.astype(dtype) for d in net.data]
         return SlicedContraction(net.indices, net.shapes, data, path, sliced, batch=batch,
                                  dtype=dtype, precision=prec, ctx=ctx, use_graph=False)
@@ -434,13 +434,17 @@ def run_c2(args):
         f"{exec_flops * steps32 / ms32 / 1e9:.1f} TFLOP/s")
     del sc32
 
-    # FP64 reference of the first execution's slices (complex128, FP64 MFMA).
-    sc64 = build("fp64", np.complex128)
+    # FP64 reference (complex128, FP64 MFMA) of the first execution's first
+    # quarter of slices: two fewer sliced indices batched keep its arena (16-byte
+    # elements) at half the complex64 one; the slice order is the same, so
+    # its first execution holds the first slices of the complex64 runs.
+    batch64 = batch[2:] if len(batch) >= 2 else batch
+    sc64 = build("fp64", np.complex128, batch64)
+    n_ref = sc64.n_batch
     ms64, _ = timed(sc64, 1, 0, False)
-    res64 = sc64.slice_results(n_batch)
-    sum64 = sc64.reduce(n_batch)
+    res64 = sc64.slice_results(n_ref)
+    sum64 = sc64.reduce(n_ref)
     del sc64
-    n_ref = n_batch
     err16 = float(np.linalg.norm(res16[:n_ref] - res64) / np.linalg.norm(res64))
     err32 = float(np.linalg.norm(res32[:n_ref] - res64) / np.linalg.norm(res64))
     amp16 = complex(np.sum(res16[:n_ref].astype(np.complex128)))
@@ -502,7 +506,8 @@ def run_c2(args):
                             "fp32_partial_amplitude": amp_err32,
                             "slices_compared": n_ref},
         "other_split_modes": compare,
-        "fp64_mfma": {"ms_per_step": round(ms64, 3)},
+        "refined_path": c2_refined_leg(args, ctx, net, label) if args.refined else None,
+        "fp64_mfma": {"ms_per_step": round(ms64, 3), "slices_per_step": n_ref},
         "kernels_split": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
                           for k, v in prof16.items()},
         "kernels_fp32": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
@@ -515,6 +520,82 @@ def run_c2(args):
                      "traffic": profiled_traffic("c2_sliced_amplitude.md", dname),
                      "traffic_source": "profiles/c2_sliced_amplitude.md"},
     }, (plan, net)
+
+
+C2_REFINED_PLAN = os.path.join(ROOT, "jet_amd", "data", "grid_7x7x14_seed2026_s12_refined.json")
+
+
+def c2_refined_leg(args, ctx, net, label):
+    """The same amplitude along the refined path (the min-fill path after
+    jet_amd.paths.refine_sliced_path, `python -m jet_amd.paths --refine`):
+    about 4x fewer flops per slice, with more of its steps bound by memory
+    traffic than by the MFMA pipes.  Split and FP32 runs timed as one
+    captured graph each; errors against a complex128 FP64-MFMA run of the
+    first execution's first quarter of slices."""
+    import json
+    from jet_amd import _native as nat
+    from jet_amd.network import SlicedContraction
+    with open(C2_REFINED_PLAN) as f:
+        plan = json.load(f)
+    path, sliced = [tuple(p) for p in plan["path"]], plan["sliced"]
+    batch = sliced[len(sliced) - args.batch_log2:] if args.batch_log2 else []
+
+    def build(prec, dtype, b):
+        data = [d.astype(dtype) for d in net.data]
+        return SlicedContraction(net.indices, net.shapes, data, path, sliced, batch=b,
+                                 dtype=dtype, precision=prec, ctx=ctx, use_graph=False)
+
+    def timed(sc, iters):
+        wb = sc.capture(range(1))
+        wb.launch()
+        ctx.synchronize()
+        wb.close()
+        tb = sc.capture(range(1, 1 + iters))
+        ctx.synchronize()
+        e0, e1 = nat.Event(ctx), nat.Event(ctx)
+        e0.record()
+        tb.launch()
+        e1.record()
+        ctx.synchronize()
+        ms = e0.elapsed_ms(e1)
+        tb.close()
+        return ms
+
+    steps = args.steps
+    sc = build(SPLIT_MODES[args.split][1], np.complex64, batch)
+    n_batch, exec_flops = sc.n_batch, sc.flops_per_execution
+    ms16 = timed(sc, steps)
+    res16 = sc.slice_results(n_batch)
+    sc.net.close()
+    del sc
+    sc = build("fp32", np.complex64, batch)
+    ms32 = timed(sc, 1)
+    res32 = sc.slice_results(n_batch)
+    sc.net.close()
+    del sc
+    sc = build("fp64", np.complex128, batch[2:] if len(batch) >= 2 else batch)
+    n_ref = sc.n_batch
+    sc.run([0])
+    res64 = sc.slice_results(n_ref)
+    sc.net.close()
+    del sc
+    err = lambda r: float(np.linalg.norm(r[:n_ref] - res64) / np.linalg.norm(res64))
+    sps16 = steps * n_batch / (ms16 * 1e-3)
+    sps32 = n_batch / (ms32 * 1e-3)
+    log(f"c2 refined path: {label} {ms16 / steps:.1f} ms/execution ({sps16:.1f} slices/s, "
+        f"{exec_flops * steps / ms16 / 1e9:.1f} TFLOP/s), fp32 {ms32:.1f} ms/execution; "
+        f"rel err per-slice {err(res16):.2e}, fp32 {err(res32):.2e}")
+    return {"plan": os.path.relpath(C2_REFINED_PLAN, ROOT),
+            "per_slice_pathinfo_flops": plan["per_slice_pathinfo_flops"],
+            "per_slice_max_log2_size": plan["per_slice_max_log2_size"],
+            "slices_per_step": n_batch, "ms_per_step": round(ms16 / steps, 3),
+            "slices_per_s": round(sps16, 4),
+            "tflops": round(exec_flops * steps / (ms16 * 1e-3) / 1e12, 3),
+            "projected_amplitude_s": round(4096 / sps16, 1),
+            "fp32_mfma": {"ms_per_step": round(ms32, 3), "slices_per_s": round(sps32, 4)},
+            "speedup_split_vs_fp32": round(sps16 / sps32, 3),
+            "rel_err_vs_fp64": {"split_per_slice": err(res16), "fp32_per_slice": err(res32),
+                                "slices_compared": n_ref}}
 
 
 def c2_cpu_baseline(host, threads, budget_s=20.0):
@@ -741,6 +822,8 @@ def main():
     ap.add_argument("--steps32", type=int, default=1,
                     help="c2: executions timed in plain FP32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-refined", dest="refined", action="store_false",
+                    help="c2: skip the refined-path leg")
     ap.add_argument("--eager", action="store_true",
                     help="c2: launch the timed executions eagerly instead of as one "
                          "captured hipGraph (for kernel tracers)")

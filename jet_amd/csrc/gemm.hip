@@ -242,6 +242,84 @@ __global__ __launch_bounds__(256) void cgemm_small_kernel(GemmArgs g)
     }
 }
 
+// Short-K products (K <= kShortK) with many outputs are bound by writing C.
+// Column form: a block owns up to kShortRows rows of one batch and 256
+// consecutive columns; each thread keeps its column's K values of B in
+// registers, reads the (wave-uniform) A values of each row and writes one
+// output per row, consecutive threads to consecutive addresses.  No
+// division per output.
+constexpr int kShortK = 16;
+constexpr int kShortRows = 32;
+
+template <class R>
+__global__ __launch_bounds__(256) void cgemm_shortk_cols_kernel(GemmArgs g, int64_t m_tiles,
+                                                                int64_t n_tiles)
+{
+    using C2 = typename Vec<R>::c;
+    const int64_t t = blockIdx.x;
+    const int64_t nt = t % n_tiles, rest = t / n_tiles;
+    const int64_t mt = rest % m_tiles, b = rest / m_tiles;
+    const int64_t n = nt * 256 + threadIdx.x;
+    if (n >= g.N) return;
+    const int K = int(g.K);
+    const C2 *bb = static_cast<const C2 *>(g.B) + b * g.sB + n * g.ldb;
+    C2 y[kShortK];
+#pragma unroll
+    for (int k = 0; k < kShortK; k++) y[k] = k < K ? bb[k] : C2{0, 0};
+    const C2 *A = static_cast<const C2 *>(g.A) + b * g.sA;
+    C2 *C = static_cast<C2 *>(g.C) + b * g.sC + n;
+    const int64_t m_end = min(g.M, (mt + 1) * int64_t(kShortRows));
+    for (int64_t m = mt * kShortRows; m < m_end; m++) {
+        const C2 *a = A + m * g.lda;
+        R sr = 0, si = 0;
+#pragma unroll
+        for (int k = 0; k < kShortK; k++) {
+            if (k < K) {
+                const C2 x = a[k];
+                sr = fma(x.x, y[k].x, sr);
+                sr = fma(-x.y, y[k].y, sr);
+                si = fma(x.x, y[k].y, si);
+                si = fma(x.y, y[k].x, si);
+            }
+        }
+        C[m * g.ldc] = C2{sr, si};
+    }
+}
+
+// Row form, for few columns: a thread owns one row (its K values of A in
+// registers) and writes that row's N outputs; B's values are wave-uniform.
+template <class R>
+__global__ __launch_bounds__(256) void cgemm_shortk_rows_kernel(GemmArgs g, int64_t m_tiles)
+{
+    using C2 = typename Vec<R>::c;
+    const int64_t t = blockIdx.x;
+    const int64_t mt = t % m_tiles, b = t / m_tiles;
+    const int64_t m = mt * 256 + threadIdx.x;
+    if (m >= g.M) return;
+    const int K = int(g.K);
+    const C2 *a = static_cast<const C2 *>(g.A) + b * g.sA + m * g.lda;
+    C2 x[kShortK];
+#pragma unroll
+    for (int k = 0; k < kShortK; k++) x[k] = k < K ? a[k] : C2{0, 0};
+    const C2 *B = static_cast<const C2 *>(g.B) + b * g.sB;
+    C2 *C = static_cast<C2 *>(g.C) + b * g.sC + m * g.ldc;
+    for (int64_t n = 0; n < g.N; n++) {
+        const C2 *bb = B + n * g.ldb;
+        R sr = 0, si = 0;
+#pragma unroll
+        for (int k = 0; k < kShortK; k++) {
+            if (k < K) {
+                const C2 y = bb[k];
+                sr = fma(x[k].x, y.x, sr);
+                sr = fma(-x[k].y, y.y, sr);
+                si = fma(x[k].x, y.y, si);
+                si = fma(x[k].y, y.x, si);
+            }
+        }
+        C[n] = C2{sr, si};
+    }
+}
+
 template <class R>
 __global__ __launch_bounds__(256) void cgemm_smallred_kernel(GemmArgs g)
 {
@@ -1103,6 +1181,20 @@ void launch_valu(int kernel, const GemmArgs &g, hipStream_t stream)
         hipLaunchKernelGGL(cgemm_smallred_kernel<R>, dim3((unsigned)total), dim3(256), 0,
                            stream, g);
     }
+    else if (g.K <= kShortK && g.N >= 64 && total >= (int64_t(1) << 16)) {
+        const int64_t m_tiles = ceil_div(g.M, kShortRows), n_tiles = ceil_div(g.N, 256);
+        const int64_t blocks = g.batch * m_tiles * n_tiles;
+        JH_REQUIRE(blocks <= 0x7fffffff, "Too many blocks.");
+        hipLaunchKernelGGL(cgemm_shortk_cols_kernel<R>, dim3((unsigned)blocks), dim3(256), 0,
+                           stream, g, m_tiles, n_tiles);
+    }
+    else if (g.K <= kShortK && g.M >= 1024 && g.N < 64) {
+        const int64_t m_tiles = ceil_div(g.M, 256);
+        const int64_t blocks = g.batch * m_tiles;
+        JH_REQUIRE(blocks <= 0x7fffffff, "Too many blocks.");
+        hipLaunchKernelGGL(cgemm_shortk_rows_kernel<R>, dim3((unsigned)blocks), dim3(256), 0,
+                           stream, g, m_tiles);
+    }
     else {
         const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
         hipLaunchKernelGGL(cgemm_small_kernel<R>, dim3(grid), dim3(256), 0, stream, g);
@@ -1134,7 +1226,6 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
     if (kernel == kGemmF32Mfma) {
         JH_REQUIRE(elem_bytes == 8, "FP32 MFMA kernel needs complex64 data.");
         constexpr int BM = 128, BN = 128;
-        JH_REQUIRE(ceil_div(g.N, BN) <= 65535, "N too large.");
         JH_REQUIRE(ceil_div(g.M, BM) * ceil_div(g.N, BN) <= 0x7fffffff, "Too many tiles.");
         dim3 grid((unsigned)(ceil_div(g.M, BM) * ceil_div(g.N, BN)), 1, (unsigned)g.batch);
         prof::Scope ps(prof::kGemmF32, stream, gemm_flops(g), gemm_bytes(g, 8));
@@ -1144,7 +1235,6 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
     else if (kernel == kGemmF64Mfma) {
         JH_REQUIRE(elem_bytes == 16, "FP64 MFMA kernel needs complex128 data.");
         constexpr int BM = 128, BN = 64;
-        JH_REQUIRE(ceil_div(g.N, BN) <= 65535, "N too large.");
         JH_REQUIRE(ceil_div(g.M, BM) * ceil_div(g.N, BN) <= 0x7fffffff, "Too many tiles.");
         dim3 grid((unsigned)(ceil_div(g.M, BM) * ceil_div(g.N, BN)), 1, (unsigned)g.batch);
         prof::Scope ps(prof::kGemmF64, stream, gemm_flops(g), gemm_bytes(g, 16));

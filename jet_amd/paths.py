@@ -448,6 +448,266 @@ def choose_slices_for_flops(indices, dims, path: Path, n_slices: int,
     return chosen
 
 
+class _Tree:
+    """A contraction tree over label bitmasks (labels of a network whose
+    every index is on at most two tensors: a contraction's result is the
+    symmetric difference of its operands)."""
+
+    def __init__(self, indices, dims, path: Path, sliced=(), memory_weight: float = 0.0,
+                 step_cost=None):
+        self.memory_weight = float(memory_weight)
+        self.step_cost = step_cost
+        sl = set(sliced)
+        labels = sorted({l for ix in indices for l in ix if l not in sl})
+        bit = {l: i for i, l in enumerate(labels)}
+        self.log2dim = [math.log2(dims[l]) for l in labels]
+        self.unit = all(dims[l] == 2 for l in labels)
+        self.n = len(indices)
+        self.mask: Dict[int, int] = {}
+        for i, ix in enumerate(indices):
+            m = 0
+            for l in ix:
+                if l not in sl:
+                    m |= 1 << bit[l]
+            self.mask[i] = m
+        self.kids: Dict[int, Tuple[int, int]] = {}
+        self.next_id = self.n
+        for (a, b) in path:
+            z = self.next_id
+            self.next_id += 1
+            self.kids[z] = (a, b)
+            self.mask[z] = self.mask[a] ^ self.mask[b]
+        self.root = self.next_id - 1
+
+    def log2size(self, m: int) -> float:
+        if self.unit:
+            return float(m.bit_count())
+        return sum(self.log2dim[i] for i in range(m.bit_length()) if m >> i & 1)
+
+    def cost(self, a_mask: int, b_mask: int) -> float:
+        """PathInfo flops of one contraction, |out| * 2 * prod(contracted),
+        plus memory_weight flops per element read or written (|a| + |b| +
+        |out|): the exchange rate between the two on the target device."""
+        if self.step_cost is not None:
+            return self.step_cost(self.log2size(a_mask & ~b_mask), self.log2size(b_mask & ~a_mask),
+                                  self.log2size(a_mask & b_mask))
+        c = 2.0 * 2.0 ** self.log2size(a_mask | b_mask)
+        if self.memory_weight:
+            c += self.memory_weight * (2.0 ** self.log2size(a_mask) + 2.0 ** self.log2size(b_mask)
+                                       + 2.0 ** self.log2size(a_mask ^ b_mask))
+        return c
+
+    def node_cost(self, z: int) -> float:
+        a, b = self.kids[z]
+        return self.cost(self.mask[a], self.mask[b])
+
+    def total(self) -> float:
+        return sum(self.node_cost(z) for z in self.kids)
+
+    def max_log2size(self) -> float:
+        return max(self.log2size(self.mask[z]) for z in self.kids)
+
+    def to_path(self) -> Path:
+        """SSA path by post-order traversal (larger-cost child first)."""
+        order: List[int] = []
+        stack = [(self.root, False)]
+        while stack:
+            z, done = stack.pop()
+            if z < self.n:
+                continue
+            if done:
+                order.append(z)
+                continue
+            stack.append((z, True))
+            a, b = self.kids[z]
+            ca = self.subtree_cost(a)
+            cb = self.subtree_cost(b)
+            first, second = (a, b) if ca >= cb else (b, a)
+            stack.append((second, False))
+            stack.append((first, False))
+        ssa = {i: i for i in range(self.n)}
+        path: Path = []
+        for z in order:
+            a, b = self.kids[z]
+            path.append((ssa[a], ssa[b]))
+            ssa[z] = self.n + len(path) - 1
+        return path
+
+    def subtree_cost(self, z: int) -> float:
+        if z < self.n:
+            return 0.0
+        total, stack = 0.0, [z]
+        while stack:
+            x = stack.pop()
+            if x < self.n:
+                continue
+            total += self.node_cost(x)
+            stack.extend(self.kids[x])
+        return total
+
+
+def _optimal_subtree(masks: List[int], tree: _Tree, max_log2: float):
+    """Cheapest contraction tree over `masks` (dynamic programming over
+    subsets); intermediates above max_log2 are not allowed.  Returns
+    (cost, split) where split[S] = the first part of subset S's best split."""
+    k = len(masks)
+    full = (1 << k) - 1
+    res = [0] * (1 << k)
+    for S in range(1, 1 << k):
+        low = S & -S
+        res[S] = res[S ^ low] ^ masks[low.bit_length() - 1]
+    inf = float("inf")
+    best = [inf] * (1 << k)
+    split = [0] * (1 << k)
+    for i in range(k):
+        best[1 << i] = 0.0
+    for S in range(1, 1 << k):
+        if S & (S - 1) == 0:
+            continue
+        if S != full and tree.log2size(res[S]) > max_log2:
+            continue
+        low = S & -S
+        rest = S ^ low
+        # A holds S's lowest element: each unordered split once
+        sub = rest
+        b_best, b_split = inf, 0
+        while True:
+            A = sub | low
+            B = S ^ A
+            if B and best[A] < inf and best[B] < inf:
+                c = best[A] + best[B] + tree.cost(res[A], res[B])
+                if c < b_best:
+                    b_best, b_split = c, A
+            if sub == 0:
+                break
+            sub = (sub - 1) & rest
+        best[S] = b_best
+        split[S] = b_split
+    return best[full], split
+
+
+def reconfigure_path(indices, dims, path: Path, sliced: Sequence[str] = (),
+                     max_log2_size: float = float("inf"), subtree_size: int = 8,
+                     passes: int = 4, max_nodes: Optional[int] = None,
+                     memory_weight: float = 0.0, step_cost=None) -> Path:
+    """Subtree reconfiguration of a contraction path, for the network with
+    `sliced` indices removed: for every internal node, the subtree below it
+    cut at `subtree_size` frontier tensors (expanding the costliest
+    contractions first) is replaced by the cheapest contraction order over
+    that frontier when that is cheaper (exact dynamic programming over
+    subsets), keeping every intermediate within max_log2_size.  Repeated for
+    `passes` passes or until nothing improves; with max_nodes, each pass
+    visits only that many of the costliest nodes.  The cost of a contraction
+    is its PathInfo flops plus memory_weight per element of its operands and
+    result (0: flops only; on a GPU whose memory-bound steps run at B bytes/s
+    and MFMA steps at F flop/s, about 8 * F / B / 4 for complex64).  A
+    step_cost(log2 M, log2 N, log2 K) callable replaces that cost (e.g. a
+    time model of the kernels that will run each step).  Needs every index
+    on at most two tensors (circuit networks)."""
+    counts: Dict[str, int] = {}
+    for ix in indices:
+        for l in ix:
+            counts[l] = counts.get(l, 0) + 1
+    if any(c > 2 for c in counts.values()):
+        raise ValueError("reconfigure_path needs every index on at most two tensors.")
+    tree = _Tree(indices, dims, path, sliced, memory_weight, step_cost)
+    for _ in range(passes):
+        improved = False
+        nodes = sorted(tree.kids, key=tree.node_cost, reverse=True)
+        if max_nodes is not None:
+            nodes = nodes[:max_nodes]
+        for v in nodes:
+            if v not in tree.kids:
+                continue
+            # frontier: expand the costliest internal node until subtree_size
+            frontier = list(tree.kids[v])
+            inner = [v]
+            while len(frontier) < subtree_size:
+                cand = [x for x in frontier if x >= tree.n]
+                if not cand:
+                    break
+                x = max(cand, key=tree.node_cost)
+                frontier.remove(x)
+                frontier.extend(tree.kids[x])
+                inner.append(x)
+            if len(inner) < 2:
+                continue
+            old = sum(tree.node_cost(z) for z in inner)
+            masks = [tree.mask[x] for x in frontier]
+            new, split = _optimal_subtree(masks, tree, max_log2_size)
+            if not new < old * (1 - 1e-12):
+                continue
+            # rebuild: v keeps its id (same result), fresh ids for the rest
+            for z in inner[1:]:
+                del tree.kids[z]
+                del tree.mask[z]
+
+            def build(S, node_id=None):
+                if S & (S - 1) == 0:
+                    return frontier[S.bit_length() - 1]
+                A = split[S]
+                a = build(A)
+                b = build(S ^ A)
+                z = node_id if node_id is not None else tree.next_id
+                if node_id is None:
+                    tree.next_id += 1
+                tree.kids[z] = (a, b)
+                tree.mask[z] = tree.mask[a] ^ tree.mask[b]
+                return z
+
+            build((1 << len(frontier)) - 1, v)
+            improved = True
+        if not improved:
+            break
+    return tree.to_path()
+
+
+def mi355x_step_seconds(log2_m: float, log2_n: float, log2_k: float,
+                        operand_bytes: float = 40.0, output_bytes: float = 16.0,
+                        launch_s: float = 8e-6, batch_bits: float = 6.0) -> float:
+    """Roofline time model of one complex64 pairwise contraction (one slice)
+    on this library's MI355X kernels, for path refinement:
+
+    * M, N, K >= 128: the split-FP16 4M GEMM at ~470 TF/s algorithmic, plus
+      operand preparation (permutation, row maximum, split: about
+      `operand_bytes` per operand element) and the output write (and its
+      permutation, `output_bytes` per element) at ~3 TB/s;
+    * other MFMA shapes (M >= 32, N, K >= 16): the FP32 MFMA kernel with
+      128x128 tiles (tile waste counted) at ~110 TF/s, or its operand and
+      output traffic at ~3 TB/s (x1.5 for permutations), whichever is larger;
+    * the rest: VALU kernels at ~20 TF/s or 2.5 TB/s;
+    * plus `launch_s` per step, shared by the 2**batch_bits slices of one
+      batched execution."""
+    lm, ln, lk = log2_m, log2_n, log2_k
+    fl = 8.0 * 2.0 ** (lm + ln + lk)
+    a, b, c = 2.0 ** (lm + lk), 2.0 ** (ln + lk), 2.0 ** (lm + ln)
+    if lm >= 7 and ln >= 7 and lk >= 7:
+        t = fl / 470e12 + (operand_bytes * (a + b) + output_bytes * c) / 3e12
+    elif lm >= 5 and ln >= 4 and lk >= 4:
+        padded = 8.0 * 2.0 ** (max(lm, 7) + max(ln, 7) + lk)
+        t = max(padded / 110e12, 1.5 * 8.0 * (a + b + c) / 3e12)
+    else:
+        t = max(fl / 20e12, 1.5 * 8.0 * (a + b + c) / 2.5e12)
+    return t + launch_s / 2.0 ** batch_bits
+
+
+# Two refinement stages: output-light model first, then the full operand
+# preparation cost (the second stage alone stops in a worse local optimum).
+REFINE_STAGES = ((12.0, 0.0, 7.0), (40.0, 16.0, 6.0))
+
+
+def refine_sliced_path(indices, dims, path: Path, sliced: Sequence[str], max_log2_size: float,
+                       subtree_size: int = 10, stages=REFINE_STAGES) -> Path:
+    """Subtree reconfiguration of a sliced path against mi355x_step_seconds,
+    one pass set per (operand_bytes, output_bytes, batch_bits) stage."""
+    for ob, cb, bb in stages:
+        def cost(lm, ln, lk, ob=ob, cb=cb, bb=bb):
+            return mi355x_step_seconds(lm, ln, lk, ob, cb, batch_bits=bb)
+        path = reconfigure_path(indices, dims, path, sliced, max_log2_size=max_log2_size,
+                                subtree_size=subtree_size, passes=20, step_cost=cost)
+    return path
+
+
 def _sliced_trial(args):
     indices, dims, t, seed, n_slices, max_log2_size = args
     p = minfill_trial(indices, dims, t, seed)
@@ -528,7 +788,43 @@ if __name__ == "__main__":
     ap.add_argument("--trials", type=int, default=64)
     ap.add_argument("--max-log2-size", type=float, default=29.0)
     ap.add_argument("--workers", type=int, default=1)
+    ap.add_argument("--refine", metavar="PLAN_JSON",
+                    help="refine an existing plan's path (refine_sliced_path) and order its "
+                         "sliced indices so that the last ones batch within --batch-gib")
+    ap.add_argument("--batch-gib", type=float, default=150.0)
+    ap.add_argument("--out", help="where --refine writes (default: over its input)")
     a = ap.parse_args()
+    if a.refine:
+        from . import circuits
+        from .network import choose_batch
+        with open(a.refine) as f:
+            plan = json.load(f)
+        circ = circuits.random_grid_circuit(plan["rows"], plan["cols"], plan["depth"],
+                                            seed=plan["seed"])
+        net = circuits.amplitude_network(circ)
+        dims = {l: 2 for ix in net.indices for l in ix}
+        mx = plan["per_slice_max_log2_size"]
+        path = refine_sliced_path(net.indices, dims, [tuple(p) for p in plan["path"]],
+                                  plan["sliced"], mx)
+        batch = choose_batch(net.indices, net.shapes, path, plan["sliced"], a.batch_gib * 2**30)
+        sliced = [l for l in plan["sliced"] if l not in batch] + batch
+        full = path_stats(net.indices, dims, path)
+        per = path_stats(net.indices, dims, path, sliced=sliced)
+        plan.update({"path": [list(p) for p in path], "sliced": sliced,
+                     "refined": {"method": "reconfigure_path against mi355x_step_seconds",
+                                 "stages": [list(x) for x in REFINE_STAGES], "subtree_size": 10,
+                                 "batched_indices": len(batch), "batch_budget_gib": a.batch_gib,
+                                 "unrefined_per_slice_pathinfo_flops":
+                                     plan["per_slice_pathinfo_flops"]},
+                     "pathinfo_flops": full.total_flops, "max_log2_size": full.max_log2_size,
+                     "per_slice_pathinfo_flops": per.total_flops,
+                     "per_slice_max_log2_size": per.max_log2_size})
+        out = a.out or a.refine
+        with open(out, "w") as f:
+            json.dump(plan, f)
+        print(out, "per slice %.3e max %d, %d batched indices" % (
+            per.total_flops, per.max_log2_size, len(batch)))
+        raise SystemExit(0)
     plan = grid_circuit_plan(a.rows, a.cols, a.depth, a.seed, a.slices, a.trials, a.max_log2_size,
                              a.workers)
     d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")

@@ -167,6 +167,11 @@ def test_contract_random_small(ctx, prec, dtype):
     (nat.PREC_FP64, np.complex128, (256, 128, 96)),
     (nat.PREC_FP64, np.complex128, (130, 70, 50)),
     (nat.PREC_FP32, np.complex64, (8, 4, 4000)),
+    (nat.PREC_FP32, np.complex64, (300, 1000, 5)),      # short K, column form
+    (nat.PREC_FP32, np.complex64, (20, 70000, 16)),
+    (nat.PREC_FP32, np.complex64, (5000, 7, 3)),        # short K, row form
+    (nat.PREC_FP64, np.complex128, (70, 3000, 12)),
+    (nat.PREC_FP64, np.complex128, (3000, 40, 1)),
     (nat.PREC_FP32, np.complex64, (1, 1, 1 << 20)),
     (nat.PREC_FP32, np.complex64, (3, 5, 100000)),
     (nat.PREC_FP64, np.complex128, (2, 1, 1 << 18)),
@@ -191,6 +196,23 @@ def test_gemm_kernels(ctx, prec, dtype, shape):
         assert info["kernel"] == "bf16x3_4m_mfma"
     if prec == nat.PREC_F16X3_4M and min(shape) >= 128:
         assert info["kernel"] == "f16x3_4m_mfma"
+
+
+def test_gemm_more_than_2_23_columns(ctx):
+    """An FP32 MFMA GEMM whose N needs more than 65535 column tiles (the
+    tile index is flattened): a short, very wide product, with a ragged last
+    tile; sampled columns against numpy."""
+    M, K, N = 32, 16, (1 << 23) + 77
+    rng = np.random.default_rng(31)
+    a = rand_c(rng, M * K, np.complex64)
+    b = rand_c(rng, K * N, np.complex64)
+    say(f"gemm wide N={N}")
+    got, info = contract(ctx, a, [0, 1], [M, K], b, [1, 2], [K, N], [0, 2])
+    assert info["kernel"] == "fp32_mfma", info
+    got = got.reshape(M, N)
+    cols = np.concatenate([np.arange(64), rng.integers(0, N, 256), np.arange(N - 200, N)])
+    want = a.reshape(M, K).astype(np.complex128) @ b.reshape(K, N)[:, cols].astype(np.complex128)
+    assert rel_err(got[:, cols], want) < 1e-5
 
 
 def test_batch_modes(ctx):
@@ -317,6 +339,20 @@ def test_reduce_compensated(ctx):
     got = from_device(out, 2, np.float64)
     want = oracle.sum_compensated(x)
     assert abs(got[0] - want.real) < 1e-6 and abs(got[1] - want.imag) < 1e-6
+
+
+def test_batch_modes_short_k(ctx):
+    """Batched short-K products on the VALU kernels (column and row forms)."""
+    rng = np.random.default_rng(14)
+    for (S, M, K, N) in [(3, 100, 4, 700), (2, 3000, 6, 9)]:
+        a = rand_c(rng, S * M * K, np.complex64)
+        b = rand_c(rng, K * S * N, np.complex64)
+        say(f"batch modes short K {(S, M, K, N)}")
+        got, info = contract(ctx, a, [0, 1, 2], [S, M, K], b, [2, 0, 3], [K, S, N], [0, 1, 3])
+        want = np.einsum("smk,ksn->smn", a.reshape(S, M, K).astype(np.complex128),
+                         b.reshape(K, S, N).astype(np.complex128)).reshape(-1)
+        assert info["batch"] == S and info["kernel"] == "valu", info
+        assert rel_err(got, want) < 1e-5, (info, rel_err(got, want))
 
 
 @pytest.mark.parametrize("prec", [nat.PREC_BF16X3, nat.PREC_BF16X3_4M, nat.PREC_F16X3_4M,

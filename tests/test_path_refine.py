@@ -1,0 +1,93 @@
+"""Subtree reconfiguration of contraction paths (jet_amd.paths).
+
+* On a 6-tensor network, reconfiguring the whole tree (subtree size >= the
+  number of tensors) reaches the cheapest of all contraction trees, found
+  here by brute force over every pairwise order.
+* On a 4x4 depth-8 circuit amplitude: the refined path is a valid SSA path
+  of the same network, never costs more than the input path under the
+  cost it minimises, and respects the intermediate-size limit.
+* The MI355X step-time model charges the split GEMM for large M, N, K and
+  memory traffic elsewhere.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from jet_amd import circuits, paths
+
+
+def _cost_of(indices, dims, path):
+    return paths.path_stats(indices, dims, path).total_flops
+
+
+def _brute_force_best(indices, dims):
+    """Cheapest PathInfo flops over all contraction orders (tiny networks)."""
+    best = math.inf
+
+    def rec(nodes, cost):
+        nonlocal best
+        if cost >= best:
+            return
+        if len(nodes) == 1:
+            best = cost
+            return
+        for i, j in itertools.combinations(range(len(nodes)), 2):
+            a, b = nodes[i], nodes[j]
+            out = a ^ b
+            c = math.prod(dims[l] for l in out) * 2 * math.prod(dims[l] for l in a & b)
+            rest = [n for k, n in enumerate(nodes) if k not in (i, j)] + [out]
+            rec(rest, cost + c)
+
+    rec([frozenset(ix) for ix in indices], 0)
+    return best
+
+
+def test_whole_tree_reconfiguration_is_optimal():
+    rng = np.random.default_rng(7)
+    # a ring of 6 tensors with a chord, random bond dimensions
+    indices = [["a", "b", "g"], ["b", "c"], ["c", "d", "h"], ["d", "e"], ["e", "f", "g"],
+               ["f", "a", "h"]]
+    dims = {l: int(rng.integers(2, 6)) for l in "abcdefgh"}
+    path = [(0, 1), (2, 3), (4, 5), (6, 7), (8, 9)]
+    got = paths.reconfigure_path(indices, dims, path, subtree_size=8, passes=4)
+    assert _cost_of(indices, dims, got) == pytest.approx(_brute_force_best(indices, dims))
+
+
+def _valid_ssa(n, path):
+    used = set()
+    for s, (a, b) in enumerate(path):
+        assert a != b and a < n + s and b < n + s and a not in used and b not in used
+        used.update((a, b))
+    assert len(path) == n - 1
+
+
+def test_refined_circuit_path():
+    circ = circuits.random_grid_circuit(4, 4, 8, seed=3)
+    net = circuits.amplitude_network(circ)
+    dims = {l: 2 for ix in net.indices for l in ix}
+    path = paths.best_greedy_path(net.indices, dims, trials=1)
+    sliced = paths.choose_slices_for_flops(net.indices, dims, path, 3, 12)
+    before = paths.path_stats(net.indices, dims, path, sliced=sliced)
+    for cost in (None, lambda lm, ln, lk: paths.mi355x_step_seconds(lm, ln, lk)):
+        p2 = paths.reconfigure_path(net.indices, dims, path, sliced, max_log2_size=12,
+                                    subtree_size=8, passes=3, step_cost=cost)
+        _valid_ssa(len(net.indices), p2)
+        after = paths.path_stats(net.indices, dims, p2, sliced=sliced)
+        assert after.max_log2_size <= max(12, before.max_log2_size)
+        t0 = paths._Tree(net.indices, dims, path, sliced, 0.0, cost).total()
+        t1 = paths._Tree(net.indices, dims, p2, sliced, 0.0, cost).total()
+        assert t1 <= t0 * (1 + 1e-12)
+        if cost is None:
+            assert after.total_flops <= before.total_flops
+
+
+def test_step_time_model():
+    big = paths.mi355x_step_seconds(14, 14, 14)
+    gemm = 8 * 2.0 ** 42 / 470e12           # the MFMA time of a 2^14-cube GEMM
+    assert gemm < big < 1.2 * gemm           # plus operand preparation
+    # a K=2 outer product of the same output size is memory-bound, far cheaper
+    # in flops but not free
+    outer = paths.mi355x_step_seconds(14, 14, 1)
+    assert outer < big and outer > 8 * 2.0 ** 28 / 3e12
