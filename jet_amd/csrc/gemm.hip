@@ -518,6 +518,63 @@ __global__ __launch_bounds__(256) void split_f16_kernel(
     }
 }
 
+// Rows of at most kSplitOnePassK padded values: one wave per row holds the
+// whole row in registers, takes its maximum with a wave reduction and
+// writes the planes and the scale in the same pass (the operand is read
+// once instead of twice).  Same planes and scales as the two-pass path.
+constexpr int kSplitOnePassK = 2048;
+
+__global__ __launch_bounds__(256) void split_f16_rows_kernel(
+    const float2 *__restrict__ src, int64_t rows, int64_t K, int64_t ld,
+    int64_t s, SplitOperand dst)
+{
+    constexpr int CH = kSplitOnePassK / 8 / 64; // 8-value chunks per lane
+    const int lane = threadIdx.x & 63;
+    const int64_t row = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (row >= dst.batch * dst.rows_p) return;
+    const int64_t b = row / dst.rows_p, r = row % dst.rows_p;
+    const int kq_n = int(dst.k_p / 8);
+    float2 x[CH][8];
+    uint32_t m = 0u;
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+        const int kq = c * 64 + lane;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int64_t k = int64_t(kq) * 8 + e;
+            x[c][e] = (kq < kq_n && r < rows && k < K) ? src[b * s + r * ld + k]
+                                                       : float2{0.f, 0.f};
+            m = max(m, max(__float_as_uint(x[c][e].x) & 0x7fffffffu,
+                           __float_as_uint(x[c][e].y) & 0x7fffffffu));
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+    const int sh = r < rows ? row_shift(m) : 0;
+    const float sc = __builtin_ldexpf(1.f, sh);
+    if (lane == 0) dst.scale[row] = __builtin_ldexpf(1.f, -sh);
+    _Float16 *base = static_cast<_Float16 *>(dst.base);
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+        const int kq = c * 64 + lane;
+        if (kq >= kq_n) continue;
+        f16x8 o[4];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            _Float16 h, l;
+            split2h(x[c][e].x * sc, h, l);
+            o[0][e] = h;
+            o[1][e] = l;
+            split2h(x[c][e].y * sc, h, l);
+            o[2][e] = h;
+            o[3][e] = l;
+        }
+        const int64_t off = (row * (dst.k_p / 32) + kq / 4) * 128 + (kq % 4) * 8;
+#pragma unroll
+        for (int p = 0; p < 4; p++) *reinterpret_cast<f16x8 *>(base + off + p * 32) = o[p];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 3M split-BF16 complex GEMM.
 // Stage s covers product s/nk (T1, T2, T3) and k tile s%nk: it loads the hi
@@ -1272,6 +1329,17 @@ void launch_split_f16(const void *src, int64_t rows, int64_t k, int64_t ld,
     const int64_t nrow = dst.batch * dst.rows_p;
     const int64_t total = nrow * (dst.k_p / 8);
     const int64_t units = batch * rows * ceil_div(k, int64_t(kRowMaxChunk));
+    if (dst.k_p <= kSplitOnePassK) {
+        // reads the operand once, writes 4 fp16 per (padded) element
+        prof::Scope ps(prof::kSplitF16, stream, 0.0,
+                       8.0 * double(batch) * double(rows) * double(k) +
+                           8.0 * double(nrow) * double(dst.k_p));
+        JH_REQUIRE(ceil_div(nrow, 4) <= 0x7fffffff, "Too many rows.");
+        hipLaunchKernelGGL(split_f16_rows_kernel, dim3((unsigned)ceil_div(nrow, 4)), dim3(256), 0,
+                           stream, static_cast<const float2 *>(src), rows, k, ld, s, dst);
+        JH_HIP(hipGetLastError());
+        return;
+    }
     // reads the operand twice (row maxima, then the split) and writes 4 fp16
     // per (padded) element
     prof::Scope ps(prof::kSplitF16, stream, 0.0,
