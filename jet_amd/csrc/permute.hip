@@ -48,6 +48,55 @@ __global__ __launch_bounds__(kPermThreads) void permute_bits_kernel(
     }
 }
 
+// complex64 tiles that hold both input bit 0 and output bit 0: each thread
+// moves pairs of neighbouring elements, which are neighbours in the input
+// (tile element u, u + 1 with u even) and in the output (tile element v,
+// v + 1 in output order), with 16-byte loads and stores.
+__global__ __launch_bounds__(kPermThreads) void permute_bits_pairs_kernel(
+    const float2 *__restrict__ in, float2 *__restrict__ out, BitPermParams p, int64_t ntiles)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *lds = reinterpret_cast<float2 *>(smem_raw);
+    constexpr int PMAX = (1 << kTileMaxBits) / kPermThreads / 2;
+    const int tsize = 1 << p.tb;
+    const int tid = threadIdx.x;
+    int64_t ioff[PMAX], ooff[PMAX];
+    int ls0[PMAX], ls1[PMAX], ld0[PMAX], ld1[PMAX];
+#pragma unroll
+    for (int e = 0; e < PMAX; e++) {
+        const int u = 2 * (tid + e * kPermThreads);
+        int uu0 = 0, uu1 = 0;
+        ioff[e] = bits_tile_in_offset(p, u);
+        ooff[e] = bits_tile_out_offset(p, u, &uu0);
+        (void)bits_tile_out_offset(p, u + 1, &uu1);
+        ls0[e] = bits_lds_index(u, 8);
+        ls1[e] = bits_lds_index(u + 1, 8);
+        ld0[e] = bits_lds_index(uu0, 8);
+        ld1[e] = bits_lds_index(uu1, 8);
+    }
+    for (int64_t w = blockIdx.x; w < ntiles; w += gridDim.x) {
+        int64_t bin, bout;
+        bits_tile_base(p, w, &bin, &bout);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < PMAX; e++) {
+            if (2 * (tid + e * kPermThreads) < tsize) {
+                const float4 x = *reinterpret_cast<const float4 *>(in + bin + ioff[e]);
+                lds[ls0[e]] = float2{x.x, x.y};
+                lds[ls1[e]] = float2{x.z, x.w};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < PMAX; e++) {
+            if (2 * (tid + e * kPermThreads) < tsize) {
+                const float2 a = lds[ld0[e]], b = lds[ld1[e]];
+                *reinterpret_cast<float4 *>(out + bout + ooff[e]) = float4{a.x, a.y, b.x, b.y};
+            }
+        }
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void permute_generic_kernel(
     const T *__restrict__ in, T *__restrict__ out, GenericPermParams p,
@@ -79,9 +128,18 @@ static void launch_typed(const PermutePlan &plan, const void *in, void *out,
         const int64_t ntiles = int64_t(1) << plan.bits.nr;
         const int grid = (int)std::min<int64_t>(ntiles, 8192);
         const size_t lds = sizeof(T) << plan.bits.tb;
-        hipLaunchKernelGGL(permute_bits_kernel<T>, dim3(grid), dim3(kPermThreads), lds,
-                           stream, static_cast<const T *>(in), static_cast<T *>(out),
-                           plan.bits, ntiles);
+        const bool pairs = sizeof(T) == 8 && plan.bits.tb >= 9 && plan.bits.ti >= 1 &&
+                           plan.bits.tile_out[0] == 0 &&
+                           reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                           reinterpret_cast<uintptr_t>(out) % 16 == 0;
+        if (pairs)
+            hipLaunchKernelGGL(permute_bits_pairs_kernel, dim3(grid), dim3(kPermThreads), lds,
+                               stream, static_cast<const float2 *>(in),
+                               static_cast<float2 *>(out), plan.bits, ntiles);
+        else
+            hipLaunchKernelGGL(permute_bits_kernel<T>, dim3(grid), dim3(kPermThreads), lds,
+                               stream, static_cast<const T *>(in), static_cast<T *>(out),
+                               plan.bits, ntiles);
         JH_HIP(hipGetLastError());
         return;
     }
