@@ -1282,12 +1282,21 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
     JH_REQUIRE(g.batch <= 65535, "Batch count exceeds 65535.");
     if (kernel == kGemmF32Mfma) {
         JH_REQUIRE(elem_bytes == 8, "FP32 MFMA kernel needs complex64 data.");
-        constexpr int BM = 128, BN = 128;
+        // 128x128 tiles; 128x64 or 64x128 when N or M is at most 64 (half of
+        // a 128-wide tile would be padding).
+        const int BM = (g.M <= 64 && g.N > 64) ? 64 : 128, BN = g.N <= 64 ? 64 : 128;
         JH_REQUIRE(ceil_div(g.M, BM) * ceil_div(g.N, BN) <= 0x7fffffff, "Too many tiles.");
         dim3 grid((unsigned)(ceil_div(g.M, BM) * ceil_div(g.N, BN)), 1, (unsigned)g.batch);
         prof::Scope ps(prof::kGemmF32, stream, gemm_flops(g), gemm_bytes(g, 8));
-        hipLaunchKernelGGL((cgemm_embed_kernel<float, BM, BN, 16, 2, 2>), grid, dim3(256), 0,
-                           stream, g);
+        if (BN == 64)
+            hipLaunchKernelGGL((cgemm_embed_kernel<float, 128, 64, 16, 2, 2>), grid, dim3(256), 0,
+                               stream, g);
+        else if (BM == 64)
+            hipLaunchKernelGGL((cgemm_embed_kernel<float, 64, 128, 16, 2, 2>), grid, dim3(256), 0,
+                               stream, g);
+        else
+            hipLaunchKernelGGL((cgemm_embed_kernel<float, 128, 128, 16, 2, 2>), grid, dim3(256),
+                               0, stream, g);
     }
     else if (kernel == kGemmF64Mfma) {
         JH_REQUIRE(elem_bytes == 16, "FP64 MFMA kernel needs complex128 data.");

@@ -154,6 +154,8 @@ def test_contract_random_small(ctx, prec, dtype):
     (nat.PREC_FP32, np.complex64, (256, 256, 256)),
     (nat.PREC_FP32, np.complex64, (300, 70, 170)),
     (nat.PREC_FP32, np.complex64, (33, 17, 16)),
+    (nat.PREC_FP32, np.complex64, (50, 300, 40)),       # 64x128 tiles
+    (nat.PREC_FP32, np.complex64, (700, 50, 33)),       # 128x64 tiles
     (nat.PREC_BF16X3, np.complex64, (512, 512, 512)),
     (nat.PREC_BF16X3, np.complex64, (300, 200, 170)),
     (nat.PREC_BF16X3_4M, np.complex64, (512, 512, 512)),
