@@ -91,3 +91,28 @@ def test_step_time_model():
     # in flops but not free
     outer = paths.mi355x_step_seconds(14, 14, 1)
     assert outer < big and outer > 8 * 2.0 ** 28 / 3e12
+
+
+def test_committed_c2_plans_are_consistent():
+    """The committed C2 plans (min-fill, and refined) are valid SSA paths of
+    the seeded 7x7 depth-14 amplitude network; their recorded per-slice
+    PathInfo flops and sizes match path_stats; the refined one is cheaper."""
+    import json
+    import os
+    root = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "jet_amd", "data")
+    circ = circuits.random_grid_circuit(7, 7, 14, seed=2026)
+    net = circuits.amplitude_network(circ)
+    dims = {l: 2 for ix in net.indices for l in ix}
+    per = {}
+    for name in ("grid_7x7x14_seed2026_s12.json", "grid_7x7x14_seed2026_s12_refined.json"):
+        with open(os.path.join(root, name)) as f:
+            plan = json.load(f)
+        path = [tuple(p) for p in plan["path"]]
+        _valid_ssa(len(net.indices), path)
+        assert len(plan["sliced"]) == 12 and len(set(plan["sliced"])) == 12
+        st = paths.path_stats(net.indices, dims, path, sliced=plan["sliced"])
+        assert st.total_flops == plan["per_slice_pathinfo_flops"]
+        assert st.max_log2_size == plan["per_slice_max_log2_size"] <= 28
+        per[name] = st.total_flops
+    assert per["grid_7x7x14_seed2026_s12_refined.json"] < per["grid_7x7x14_seed2026_s12.json"] / 3
