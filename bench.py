@@ -68,8 +68,9 @@ SPLIT_LABEL = {"3m": "split-BF16 3M", "4m": "split-BF16 4M", "f16": "split-FP16 
 ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_x4m_wide_kernel bf16"),
                 "gemm_f16x3_4m_mfma": ("cgemm_x4m_kernel f16", "cgemm_x4m_wide_kernel f16"),
                 "gemm_bf16x3_mfma": ("cgemm_bf16x3_kernel",),
-                "gemm_fp32_mfma": ("cgemm_embed_kernel f32",),
-                "gemm_fp64_mfma": ("cgemm_embed_kernel f64",),
+                "gemm_fp32_mfma": ("cgemm_embed_kernel f32 128x128", "cgemm_embed_kernel f32 128x64",
+                                   "cgemm_embed_kernel f32 64x128"),
+                "gemm_fp64_mfma": ("cgemm_embed_kernel f64 128x64",),
                 "permute_bits": ("permute_bits_kernel c64",)}
 
 
@@ -546,19 +547,31 @@ def c2_refined_leg(args, ctx, net, label):
                                  dtype=dtype, precision=prec, ctx=ctx, use_graph=False)
 
     def timed(sc, iters):
-        wb = sc.capture(range(1))
-        wb.launch()
-        ctx.synchronize()
-        wb.close()
-        tb = sc.capture(range(1, 1 + iters))
+        """One warm-up execution, then `iters` timed ones: each a captured
+        graph (eager launches with --eager)."""
+        if args.eager:
+            sc.run(range(1))
+        else:
+            wb = sc.capture(range(1))
+            wb.launch()
+            ctx.synchronize()
+            wb.close()
         ctx.synchronize()
         e0, e1 = nat.Event(ctx), nat.Event(ctx)
-        e0.record()
-        tb.launch()
-        e1.record()
+        if args.eager:
+            e0.record()
+            sc.run(range(1, 1 + iters))
+            e1.record()
+        else:
+            tb = sc.capture(range(1, 1 + iters))
+            ctx.synchronize()
+            e0.record()
+            tb.launch()
+            e1.record()
         ctx.synchronize()
         ms = e0.elapsed_ms(e1)
-        tb.close()
+        if not args.eager:
+            tb.close()
         return ms
 
     steps = args.steps
@@ -566,6 +579,13 @@ def c2_refined_leg(args, ctx, net, label):
     n_batch, exec_flops = sc.n_batch, sc.flops_per_execution
     ms16 = timed(sc, steps)
     res16 = sc.slice_results(n_batch)
+    # one more execution under the event profiler: per-family kernel times
+    nat.Profiler.reset()
+    nat.Profiler.enable(True)
+    sc.run([0])
+    ctx.synchronize()
+    prof = nat.Profiler.read()
+    nat.Profiler.enable(False)
     sc.net.close()
     del sc
     sc = build("fp32", np.complex64, batch)
@@ -595,7 +615,25 @@ def c2_refined_leg(args, ctx, net, label):
             "fp32_mfma": {"ms_per_step": round(ms32, 3), "slices_per_s": round(sps32, 4)},
             "speedup_split_vs_fp32": round(sps16 / sps32, 3),
             "rel_err_vs_fp64": {"split_per_slice": err(res16), "fp32_per_slice": err(res32),
-                                "slices_compared": n_ref}}
+                                "slices_compared": n_ref},
+            "kernels_split": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
+                              for k, v in prof.items()}}
+
+
+def run_c2r(args):
+    """The refined-path leg of c2 alone (its JSON line: slices/s of the
+    refined path); for profiling that path."""
+    from jet_amd import _native as nat
+    plan, circ, net = c2_problem()
+    r = c2_refined_leg(args, nat.default_context(), net, SPLIT_LABEL[args.split])
+    return {"metric": METRIC, "value": r["slices_per_s"], "unit": "slices/s",
+            "ms_per_step": r["ms_per_step"],
+            "dtype": f"c64 ({SPLIT_LABEL[args.split]} MFMA on MFMA-bound steps; fp32 MFMA "
+                     "and VALU elsewhere)",
+            "config": {"workload": "c2r: the c2 amplitude along the refined path; a step is "
+                                   f"one batched execution of {r['slices_per_step']} slices",
+                       "plan": r["plan"]},
+            "refined_path": r}, (plan, net)
 
 
 def c2_cpu_baseline(host, threads, budget_s=20.0):
@@ -807,7 +845,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2r", "c3", "c4"])
     ap.add_argument("--c4-fd", type=int, default=0,
                     help="c4: angles checked by central differences (0: all)")
     ap.add_argument("--c3-slices", type=int, default=256,
@@ -847,6 +885,9 @@ def main():
         args.no_cpu_baseline = True
     elif args.workload == "c4":
         result, host = run_c4(args)
+        args.no_cpu_baseline = True
+    elif args.workload == "c2r":
+        result, host = run_c2r(args)
         args.no_cpu_baseline = True
     else:
         result, host = run_c2(args)
