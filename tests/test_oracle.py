@@ -74,7 +74,7 @@ def test_qflex_restatement_matches_reference_build():
     if oracle.ref_lib() is None:
         pytest.skip("oracle/_ref not built (reference tree absent)")
     rng = np.random.default_rng(5)
-    configs = [(64, 32), (128, 32), (256, 32), (512, 32), (1024, 32), (2048, 32), (1024, 2),
+    qflex_params = [(64, 32), (128, 32), (256, 32), (512, 32), (1024, 32), (2048, 32), (1024, 2),
                (1024, 64)]
     kinds = set()
     for t in range(120):
@@ -86,7 +86,7 @@ def test_qflex_restatement_matches_reference_build():
         perm = [int(p) for p in rng.permutation(r)]
         labels = [string.ascii_letters[i] for i in range(r)]
         new = [labels[p] for p in perm]
-        bs, md = configs[t % len(configs)]
+        bs, md = qflex_params[t % len(qflex_params)]
         n = int(np.prod(shape))
         x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
         try:
