@@ -60,3 +60,136 @@ extern "C" int check_network(int elem_bytes, int mode, int n_leaves, const int32
     }
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Emulation of a whole planned network on the host in complex128: each
+// step's operand permutations (the plan's PermutePlans, replayed by index
+// arithmetic), the batched NT product C'[b][m][n] = sum_k A'[b][m][k]
+// B'[b][n][k], then the result permuted (its C plan) or stored as is.
+// Returns 0 and the root in out, or 1 for an out-of-bounds index, 3 for a
+// planner error; with the number of permuted operand elements.
+#include <complex>
+
+namespace {
+
+using cd = std::complex<double>;
+
+int apply_permute(const PermutePlan &plan, const cd *src, cd *dst)
+{
+    const int64_t n = plan.numel;
+    if (plan.kind == PermutePlan::kCopy) {
+        for (int64_t i = 0; i < n; i++) dst[i] = src[i];
+        return 0;
+    }
+    if (plan.kind == PermutePlan::kGeneric) {
+        for (int64_t o = 0; o < n; o++) {
+            const int64_t off = generic_in_offset(plan.gen, o);
+            if (off < 0 || off >= n) return 1;
+            dst[o] = src[off];
+        }
+        return 0;
+    }
+    const BitPermParams &p = plan.bits;
+    const int tsize = 1 << p.tb;
+    const int64_t ntiles = int64_t(1) << p.nr;
+    for (int64_t w = 0; w < ntiles; w++) {
+        int64_t bin, bout;
+        bits_tile_base(p, w, &bin, &bout);
+        for (int v = 0; v < tsize; v++) {
+            int uu = 0;
+            const int64_t ooff = bits_tile_out_offset(p, v, &uu);
+            const int64_t ioff = bits_tile_in_offset(p, uu);
+            if (bin + ioff >= n || bout + ooff >= n) return 1;
+            dst[bout + ooff] = src[bin + ioff];
+        }
+    }
+    return 0;
+}
+
+} // namespace
+
+extern "C" int emu_network(int mode, int n_leaves, const int32_t *ranks, const int32_t *modes,
+                           const int64_t *exts, int n_steps, const int32_t *path, int n_out,
+                           const int32_t *out_modes, const double *const *leaves, double *out,
+                           double *permuted, char *err, int errlen)
+{
+    NetworkPlan np;
+    try {
+        np = plan_network(8, mode, n_leaves, ranks, modes, exts, n_steps, path, n_out, out_modes);
+    }
+    catch (const Error &e) {
+        std::strncpy(err, e.what(), errlen - 1);
+        err[errlen - 1] = 0;
+        return 3;
+    }
+    *permuted = 0;
+    std::vector<std::vector<cd>> node(n_leaves + n_steps);
+    size_t pos = 0;
+    for (int i = 0; i < n_leaves; i++) {
+        int64_t n = 1;
+        for (int j = 0; j < ranks[i]; j++) n *= exts[pos + j];
+        pos += ranks[i];
+        node[i].resize(n);
+        for (int64_t e = 0; e < n; e++) node[i][e] = cd(leaves[i][2 * e], leaves[i][2 * e + 1]);
+    }
+    for (int s = 0; s < n_steps; s++) {
+        const NetworkStep &st = np.steps[s];
+        const ContractionDesc &d = st.desc;
+        const std::vector<cd> &A = node[st.a], &B = node[st.b];
+        if ((int64_t)A.size() != d.size_a || (int64_t)B.size() != d.size_b) return 1;
+        std::vector<cd> At(A), Bt(B);
+        if (d.need_a) {
+            if (apply_permute(d.pa, A.data(), At.data())) return 1;
+            *permuted += double(d.size_a);
+        }
+        if (d.need_b) {
+            if (apply_permute(d.pb, B.data(), Bt.data())) return 1;
+            *permuted += double(d.size_b);
+        }
+        std::vector<cd> Cn((size_t)d.size_c), C((size_t)d.size_c);
+        for (int64_t b = 0; b < d.batch; b++)
+            for (int64_t m = 0; m < d.M; m++)
+                for (int64_t n = 0; n < d.N; n++) {
+                    cd acc = 0;
+                    const cd *a = At.data() + (b * d.M + m) * d.K;
+                    const cd *bb = Bt.data() + (b * d.N + n) * d.K;
+                    for (int64_t k = 0; k < d.K; k++) acc += a[k] * bb[k];
+                    Cn[(b * d.M + m) * d.N + n] = acc;
+                }
+        if (d.need_c) {
+            if (apply_permute(d.pc, Cn.data(), C.data())) return 1;
+        }
+        else {
+            C.swap(Cn);
+        }
+        node[n_leaves + s].swap(C);
+    }
+    const std::vector<cd> &root = n_steps ? node[n_leaves + n_steps - 1] : node[0];
+    for (size_t e = 0; e < root.size(); e++) {
+        out[2 * e] = root[e].real();
+        out[2 * e + 1] = root[e].imag();
+    }
+    return 0;
+}
+
+// Elements the plan permutes (operands and results).
+extern "C" int network_permute_stats(int mode, int n_leaves, const int32_t *ranks,
+                                     const int32_t *modes, const int64_t *exts, int n_steps,
+                                     const int32_t *path, int n_out, const int32_t *out_modes,
+                                     double *permuted)
+{
+    NetworkPlan np;
+    try {
+        np = plan_network(8, mode, n_leaves, ranks, modes, exts, n_steps, path, n_out, out_modes);
+    }
+    catch (const Error &) {
+        return 3;
+    }
+    *permuted = 0;
+    for (const NetworkStep &st : np.steps) {
+        const ContractionDesc &d = st.desc;
+        *permuted += (d.need_a ? double(d.size_a) : 0) + (d.need_b ? double(d.size_b) : 0) +
+                     (d.need_c ? double(d.size_c) : 0);
+    }
+    return 0;
+}

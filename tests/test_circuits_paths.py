@@ -11,6 +11,7 @@
   of live intermediates, before anything runs on a GPU.
 """
 import ctypes
+import string
 import os
 import subprocess
 
@@ -81,7 +82,7 @@ def _check_lib():
                           ("network_plan.cpp", "contract_plan.cpp", "permute_plan.cpp")]
     if not os.path.exists(CHECK_LIB) or os.path.getmtime(CHECK_LIB) < max(map(os.path.getmtime, srcs)):
         os.makedirs(os.path.dirname(CHECK_LIB), exist_ok=True)
-        subprocess.run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared",
+        subprocess.run(["g++", "-std=c++17", "-O2", "-fcx-limited-range", "-fPIC", "-shared",
                         "-I" + os.path.join(ROOT, "include"), "-o", CHECK_LIB] + srcs, check=True)
     return ctypes.CDLL(CHECK_LIB)
 
@@ -158,3 +159,65 @@ def test_stacked_amplitude_plan(mode):
     assert rc == 0, err
     rc1, fl1, *_ = _check(L, net.indices, net.shapes, path, [], mode=mode)
     assert rc1 == 0 and fl >= fl1
+
+
+def _emu(L, indices, shapes, data, path, out, mode):
+    lab = LabelIds()
+    ranks, modes, exts = _flatten(indices, shapes, lab)
+    o = [lab(l) for l in out]
+    flat = [int(x) for p in path for x in p]
+    arrs = [np.ascontiguousarray(np.asarray(d, np.complex128).reshape(-1)) for d in data]
+    ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    dims = dict(zip([l for ix in indices for l in ix], [s for sh in shapes for s in sh]))
+    res = np.zeros(int(np.prod([dims[l] for l in out])) if out else 1, np.complex128)
+    perm = ctypes.c_double()
+    err = ctypes.create_string_buffer(256)
+    L.emu_network.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_int]
+    r32, m32, e64, p32, o32 = (nat.i32(ranks), nat.i32(modes), nat.i64(exts), nat.i32(flat),
+                               nat.i32(o))
+    rc = L.emu_network(mode, len(indices), r32, m32, e64, len(path), p32, len(o), o32, ptrs,
+                       res.ctypes.data, ctypes.byref(perm), err, 256)
+    assert rc == 0, (rc, err.value)
+    return res, perm.value
+
+
+def test_network_plan_emulation_matches_numpy():
+    """A network plan replayed on the host (tests/native/network_check.cpp:
+    the plan's own permutation index arithmetic, the batched NT products,
+    the result permutations) equals numpy along the same path, for a network
+    with scrambled leaf mode orders, an open batch mode, and steps of the
+    shapes that run on the 128x192 split kernel (FP32 and split-FP16 plans)."""
+    L = _check_lib()
+    rng = np.random.default_rng(17)
+    # a(7 bits) c(7) n(12) g(7) h(1, open batch) labels; bond 2
+    a = [f"a{i}" for i in range(7)]
+    c = [f"c{i}" for i in range(7)]
+    nn = [f"n{i}" for i in range(12)]
+    g = [f"g{i}" for i in range(7)]
+    hb = ["h0"]
+    order = lambda ls: [ls[i] for i in rng.permutation(len(ls))]
+    indices = [order(a + c + hb), order(c + nn + hb), order(nn[:6] + g + hb),
+               order(nn[6:] + a + g + hb)]
+    shapes = [[2] * len(ix) for ix in indices]
+    data = [(rng.standard_normal(2 ** len(ix)) + 1j * rng.standard_normal(2 ** len(ix))) / 4
+            for ix in indices]
+    path = [(0, 1), (4, 2), (5, 3)]
+    out = ["h0"]
+    for mode in (0, 4):
+        got, permuted = _emu(L, indices, shapes, data, path, out, mode)
+        # numpy along the same path, pairwise (the open batch labels kept)
+        nodes = [(list(ix), d.reshape([2] * len(ix))) for ix, d in zip(indices, data)]
+        for (x, y) in path:
+            (lx, tx), (ly, ty) = nodes[x], nodes[y]
+            keep = [l for l in lx if l not in ly or l in out] + [l for l in ly if l not in lx]
+            letters = {l: string.ascii_letters[i] for i, l in enumerate(dict.fromkeys(lx + ly))}
+            spec = "".join(letters[l] for l in lx) + "," + "".join(letters[l] for l in ly) + \
+                "->" + "".join(letters[l] for l in keep)
+            nodes.append((keep, np.einsum(spec, tx, ty)))
+        lr, tr = nodes[-1]
+        want = np.transpose(tr, [lr.index(l) for l in out])
+        assert np.allclose(got.reshape(want.shape), want, rtol=1e-10, atol=1e-10), mode
+        assert permuted > 0      # the scrambled leaves are permuted
