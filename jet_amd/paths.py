@@ -451,15 +451,20 @@ def choose_slices_for_flops(indices, dims, path: Path, n_slices: int,
 class _Tree:
     """A contraction tree over label bitmasks (labels of a network whose
     every index is on at most two tensors: a contraction's result is the
-    symmetric difference of its operands)."""
+    symmetric difference of its operands, except for open labels, which are
+    never summed -- batch modes where both operands hold them)."""
 
     def __init__(self, indices, dims, path: Path, sliced=(), memory_weight: float = 0.0,
-                 step_cost=None):
+                 step_cost=None, open_labels=()):
         self.memory_weight = float(memory_weight)
         self.step_cost = step_cost
         sl = set(sliced)
         labels = sorted({l for ix in indices for l in ix if l not in sl})
         bit = {l: i for i, l in enumerate(labels)}
+        self.open = 0
+        for l in open_labels:
+            if l in bit:
+                self.open |= 1 << bit[l]
         self.log2dim = [math.log2(dims[l]) for l in labels]
         self.unit = all(dims[l] == 2 for l in labels)
         self.n = len(indices)
@@ -476,8 +481,11 @@ class _Tree:
             z = self.next_id
             self.next_id += 1
             self.kids[z] = (a, b)
-            self.mask[z] = self.mask[a] ^ self.mask[b]
+            self.mask[z] = self.result(self.mask[a], self.mask[b])
         self.root = self.next_id - 1
+
+    def result(self, a_mask: int, b_mask: int) -> int:
+        return (a_mask ^ b_mask) | (a_mask & b_mask & self.open)
 
     def log2size(self, m: int) -> float:
         if self.unit:
@@ -489,12 +497,17 @@ class _Tree:
         plus memory_weight flops per element read or written (|a| + |b| +
         |out|): the exchange rate between the two on the target device."""
         if self.step_cost is not None:
-            return self.step_cost(self.log2size(a_mask & ~b_mask), self.log2size(b_mask & ~a_mask),
-                                  self.log2size(a_mask & b_mask))
+            lm = self.log2size(a_mask & ~b_mask)
+            ln = self.log2size(b_mask & ~a_mask)
+            if not self.open:
+                return self.step_cost(lm, ln, self.log2size(a_mask & b_mask))
+            batch = a_mask & b_mask & self.open
+            return self.step_cost(lm, ln, self.log2size(a_mask & b_mask & ~self.open),
+                                  self.log2size(batch))
         c = 2.0 * 2.0 ** self.log2size(a_mask | b_mask)
         if self.memory_weight:
             c += self.memory_weight * (2.0 ** self.log2size(a_mask) + 2.0 ** self.log2size(b_mask)
-                                       + 2.0 ** self.log2size(a_mask ^ b_mask))
+                                       + 2.0 ** self.log2size(self.result(a_mask, b_mask)))
         return c
 
     def node_cost(self, z: int) -> float:
@@ -553,9 +566,15 @@ def _optimal_subtree(masks: List[int], tree: _Tree, max_log2: float):
     k = len(masks)
     full = (1 << k) - 1
     res = [0] * (1 << k)
+    union = [0] * (1 << k)
     for S in range(1, 1 << k):
         low = S & -S
-        res[S] = res[S ^ low] ^ masks[low.bit_length() - 1]
+        m = masks[low.bit_length() - 1]
+        union[S] = union[S ^ low] | m
+        res[S] = res[S ^ low] ^ m
+    if tree.open:
+        for S in range(1, 1 << k):
+            res[S] |= union[S] & tree.open
     inf = float("inf")
     best = [inf] * (1 << k)
     split = [0] * (1 << k)
@@ -589,7 +608,7 @@ def _optimal_subtree(masks: List[int], tree: _Tree, max_log2: float):
 def reconfigure_path(indices, dims, path: Path, sliced: Sequence[str] = (),
                      max_log2_size: float = float("inf"), subtree_size: int = 8,
                      passes: int = 4, max_nodes: Optional[int] = None,
-                     memory_weight: float = 0.0, step_cost=None) -> Path:
+                     memory_weight: float = 0.0, step_cost=None, open_labels=()) -> Path:
     """Subtree reconfiguration of a contraction path, for the network with
     `sliced` indices removed: for every internal node, the subtree below it
     cut at `subtree_size` frontier tensors (expanding the costliest
@@ -602,15 +621,19 @@ def reconfigure_path(indices, dims, path: Path, sliced: Sequence[str] = (),
     result (0: flops only; on a GPU whose memory-bound steps run at B bytes/s
     and MFMA steps at F flop/s, about 8 * F / B / 4 for complex64).  A
     step_cost(log2 M, log2 N, log2 K) callable replaces that cost (e.g. a
-    time model of the kernels that will run each step).  Needs every index
-    on at most two tensors (circuit networks)."""
+    time model of the kernels that will run each step).  open_labels are
+    never summed (batch modes where both operands hold them), e.g. the
+    sliced indices that a batched execution keeps; with them step_cost takes
+    a fourth argument, log2 of the step's batch extent.  Needs every other
+    index on at most two tensors (circuit networks)."""
     counts: Dict[str, int] = {}
     for ix in indices:
         for l in ix:
             counts[l] = counts.get(l, 0) + 1
-    if any(c > 2 for c in counts.values()):
-        raise ValueError("reconfigure_path needs every index on at most two tensors.")
-    tree = _Tree(indices, dims, path, sliced, memory_weight, step_cost)
+    if any(c > 2 and l not in open_labels for l, c in counts.items()):
+        raise ValueError("reconfigure_path needs every index that is not open on at most two "
+                         "tensors.")
+    tree = _Tree(indices, dims, path, sliced, memory_weight, step_cost, open_labels)
     for _ in range(passes):
         improved = False
         nodes = sorted(tree.kids, key=tree.node_cost, reverse=True)
@@ -652,7 +675,7 @@ def reconfigure_path(indices, dims, path: Path, sliced: Sequence[str] = (),
                 if node_id is None:
                     tree.next_id += 1
                 tree.kids[z] = (a, b)
-                tree.mask[z] = tree.mask[a] ^ tree.mask[b]
+                tree.mask[z] = tree.result(tree.mask[a], tree.mask[b])
                 return z
 
             build((1 << len(frontier)) - 1, v)
@@ -664,7 +687,8 @@ def reconfigure_path(indices, dims, path: Path, sliced: Sequence[str] = (),
 
 def mi355x_step_seconds(log2_m: float, log2_n: float, log2_k: float,
                         operand_bytes: float = 40.0, output_bytes: float = 16.0,
-                        launch_s: float = 8e-6, batch_bits: float = 6.0) -> float:
+                        launch_s: float = 8e-6, batch_bits: float = 6.0,
+                        log2_batch: float = 0.0) -> float:
     """Roofline time model of one complex64 pairwise contraction (one slice)
     on this library's MI355X kernels, for path refinement:
 
@@ -676,6 +700,7 @@ def mi355x_step_seconds(log2_m: float, log2_n: float, log2_k: float,
       128x128 tiles (tile waste counted) at ~110 TF/s, or its operand and
       output traffic at ~3 TB/s (x1.5 for permutations), whichever is larger;
     * the rest: VALU kernels at ~20 TF/s or 2.5 TB/s;
+    * times 2**log2_batch for a batched step (batch modes of the GEMM);
     * plus `launch_s` per step, shared by the 2**batch_bits slices of one
       batched execution."""
     lm, ln, lk = log2_m, log2_n, log2_k
@@ -688,7 +713,7 @@ def mi355x_step_seconds(log2_m: float, log2_n: float, log2_k: float,
         t = max(padded / 110e12, 1.5 * 8.0 * (a + b + c) / 3e12)
     else:
         t = max(fl / 20e12, 1.5 * 8.0 * (a + b + c) / 2.5e12)
-    return t + launch_s / 2.0 ** batch_bits
+    return t * 2.0 ** log2_batch + launch_s / 2.0 ** batch_bits
 
 
 # Two refinement stages: output-light model first, then the full operand

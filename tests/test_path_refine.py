@@ -22,8 +22,9 @@ def _cost_of(indices, dims, path):
     return paths.path_stats(indices, dims, path).total_flops
 
 
-def _brute_force_best(indices, dims):
-    """Cheapest PathInfo flops over all contraction orders (tiny networks)."""
+def _brute_force_best(indices, dims, open_labels=frozenset()):
+    """Cheapest PathInfo flops over all contraction orders (tiny networks);
+    open labels stay on the result of every contraction that holds them."""
     best = math.inf
 
     def rec(nodes, cost):
@@ -35,8 +36,8 @@ def _brute_force_best(indices, dims):
             return
         for i, j in itertools.combinations(range(len(nodes)), 2):
             a, b = nodes[i], nodes[j]
-            out = a ^ b
-            c = math.prod(dims[l] for l in out) * 2 * math.prod(dims[l] for l in a & b)
+            out = (a ^ b) | (a & b & open_labels)
+            c = math.prod(dims[l] for l in out) * 2 * math.prod(dims[l] for l in (a & b) - out)
             rest = [n for k, n in enumerate(nodes) if k not in (i, j)] + [out]
             rec(rest, cost + c)
 
@@ -53,6 +54,33 @@ def test_whole_tree_reconfiguration_is_optimal():
     path = [(0, 1), (2, 3), (4, 5), (6, 7), (8, 9)]
     got = paths.reconfigure_path(indices, dims, path, subtree_size=8, passes=4)
     assert _cost_of(indices, dims, got) == pytest.approx(_brute_force_best(indices, dims))
+
+
+def test_open_label_reconfiguration_is_optimal():
+    """Batch labels kept open (on every tensor of a batched execution that
+    holds one): the whole-tree reconfiguration still reaches the brute-force
+    optimum of the batched cost, and a time-model cost receives the batch
+    extent as its fourth argument."""
+    rng = np.random.default_rng(11)
+    indices = [["a", "b", "x"], ["b", "c", "y"], ["c", "d"], ["d", "e", "x"], ["e", "f", "y"],
+               ["f", "a", "x"]]
+    dims = {l: int(rng.integers(2, 6)) for l in "abcdefxy"}
+    path = [(0, 1), (2, 3), (4, 5), (6, 7), (8, 9)]
+    open_labels = {"x", "y"}
+    got = paths.reconfigure_path(indices, dims, path, subtree_size=8, passes=4,
+                                 open_labels=open_labels)
+    tree = paths._Tree(indices, dims, got, open_labels=open_labels)
+    assert tree.mask[tree.root] == tree.open       # only the open labels remain
+    assert tree.total() == pytest.approx(_brute_force_best(indices, dims, frozenset(open_labels)))
+    seen = []
+
+    def cost(lm, ln, lk, lb):
+        seen.append(lb)
+        return 2.0 ** (lm + ln + lk + lb)
+
+    paths.reconfigure_path(indices, dims, path, subtree_size=8, passes=1, step_cost=cost,
+                           open_labels=open_labels)
+    assert max(seen) > 0
 
 
 def _valid_ssa(n, path):
