@@ -1,16 +1,19 @@
 """Benchmark of the MI355X contraction path (one JSON line on stdout).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4]
 
 Workloads:
   c2  (default) one amplitude <b|C|0> of a seeded 7x7-qubit grid random
       circuit of depth 14 ({sqrt X, sqrt Y, sqrt W} on every qubit, then
       fSim(pi/2, pi/6) on one of four alternating nearest-neighbour edge
-      patterns per cycle), complex64, contracted along a min-fill path
-      sliced on 12 bond-2 indices (4096 slices).  The slice loop runs on the
-      GPU in batches: 2^b slices per execution (b sliced indices kept as
-      batch modes), executions over the other sliced indices.  A step is one
-      execution (2^b slices); the timed region runs --steps executions with
+      patterns per cycle), complex64, sliced on 12 bond-2 indices (4096
+      slices), contracted along the min-fill path refined for batched
+      execution (--c2-plan refined, the headline) and along the min-fill
+      path itself (the second leg; --c2-plan min-fill swaps them).  The
+      slice loop runs on the GPU in batches: 2^b slices per execution (b
+      sliced indices kept as batch modes), executions over the other sliced
+      indices.  A step is one execution (2^b slices); the timed region runs
+      --steps executions with
       the split 16-bit MFMA GEMM of --split (default split-FP16: each
       complex64 operand row scaled by a power of two, split into hi + lo
       FP16, three products per real product, FP32 accumulation), then the
@@ -26,9 +29,10 @@ Workloads:
       value); the plain FP32-MFMA path is timed beside it, and complex128 on
       FP64 MFMA gives the FP64 reference the error is measured against.
 
-value = algorithmic TFLOP/s (8*M*N*K real flops per executed GEMM) with the
-operands resident in HBM when the timed region starts; slices/s beside it.  The timed region is
-bracketed by HIP events on the library's stream and a full synchronization.
+value (c2) = slices/s, with the operands resident in HBM when the timed region
+starts; algorithmic TFLOP/s (8*M*N*K real flops per executed GEMM) beside it
+(the value of c1 and c3).  The timed region is bracketed by HIP events on the
+library's stream and a full synchronization.
 """
 from __future__ import annotations
 
@@ -337,9 +341,25 @@ C2_SEED = 2026
 C2_SLICES = 12
 
 
-def c2_problem():
+# The two C2 plans: the min-fill path sliced on 12 indices (written by
+# `python -m jet_amd.paths`), and the same path refined for batched execution
+# (`python -m jet_amd.paths --refine ... --batch-log2 7 --extra-log2 1`),
+# with the batch size each runs at by default.
+C2_PLANS = {"refined": os.path.join(ROOT, "jet_amd", "data", "grid_7x7x14_seed2026_s12_refined.json"),
+            "min-fill": os.path.join(ROOT, "jet_amd", "data", "grid_7x7x14_seed2026_s12.json")}
+C2_BATCH_LOG2 = {"refined": 8, "min-fill": 7}
+# rocprofv3 summaries (with HBM traffic) of the c2 run on each plan.
+C2_PROFILE = {"refined": "c2_sliced_amplitude.md", "min-fill": "c2_min_fill_path.md"}
+
+
+def c2_problem(plan_name="min-fill"):
     from jet_amd import circuits, paths
-    plan = paths.load_or_make_grid_plan(*C2_GRID, C2_SEED, C2_SLICES)
+    if plan_name == "min-fill":
+        plan = paths.load_or_make_grid_plan(*C2_GRID, C2_SEED, C2_SLICES)
+    else:
+        with open(C2_PLANS[plan_name]) as f:
+            plan = json.load(f)
+        plan["path"] = [tuple(p) for p in plan["path"]]
     circ = circuits.random_grid_circuit(*C2_GRID, seed=C2_SEED)
     net = circuits.amplitude_network(circ, dtype=np.complex64)
     return plan, circ, net
@@ -351,10 +371,11 @@ def run_c2(args):
 
     ctx = nat.default_context()
     t0 = time.perf_counter()
-    plan, circ, net = c2_problem()
+    plan, circ, net = c2_problem(args.c2_plan)
     path, sliced = plan["path"], plan["sliced"]
-    batch = sliced[len(sliced) - args.batch_log2:] if args.batch_log2 else []
-    log(f"c2: plan ready ({time.perf_counter() - t0:.1f}s): {len(net)} tensors, "
+    batch_log2 = C2_BATCH_LOG2[args.c2_plan] if args.batch_log2 is None else args.batch_log2
+    batch = sliced[len(sliced) - batch_log2:] if batch_log2 else []
+    log(f"c2: {args.c2_plan} plan ready ({time.perf_counter() - t0:.1f}s): {len(net)} tensors, "
         f"{len(path)} steps, per-slice PathInfo flops {plan['per_slice_pathinfo_flops']:.3e}, "
         f"batch {len(batch)} of {len(sliced)} sliced indices")
 
@@ -480,22 +501,27 @@ def run_c2(args):
     else:
         achieved = dk["bytes"] / (dk["ms"] * 1e-3)
         peak, unit, scale = PEAK["hbm"], "GB/s", 1e9
+    other = "min-fill" if args.c2_plan == "refined" else "refined"
+    path_desc = {"refined": "min-fill path refined for batched execution",
+                 "min-fill": "min-fill path"}[args.c2_plan]
     return {
         "metric": METRIC,
-        "value": round(tflops16 / 1e12, 3),
-        "unit": "TFLOP/s",
+        "value": round(sps16, 3),
+        "unit": "slices/s",
         "ms_per_step": round(ms16 / steps, 3),
         "dtype": f"c64 ({label} MFMA, three terms, fp32 accumulate, on MFMA-bound steps; "
                  "fp32 MFMA elsewhere)",
         "config": {"workload": "c2: one amplitude of a seeded 7x7-qubit grid random circuit, "
-                               "depth 14, complex64, min-fill path sliced on 12 bond-2 indices "
+                               f"depth 14, complex64, {path_desc} sliced on 12 bond-2 indices "
                                "(4096 slices); a step is one batched execution of "
                                f"{n_batch} slices",
+                   "plan": os.path.relpath(C2_PLANS[args.c2_plan], ROOT),
                    "qubits": 49, "depth": 14, "sliced_indices": len(sliced),
                    "slices_total": 4096, "slices_per_step": n_batch,
                    "tensors": len(net), "path_steps": len(path),
                    "per_slice_pathinfo_flops": plan["per_slice_pathinfo_flops"],
                    "per_slice_max_log2_size": plan["per_slice_max_log2_size"]},
+        "tflops": round(tflops16 / 1e12, 3),
         "slices_per_s": round(sps16, 4),
         "fp32_mfma": {"ms_per_step": round(ms32 / steps32, 3), "tflops": round(tflops32 / 1e12, 3),
                       "slices_per_s": round(sps32, 4)},
@@ -507,7 +533,10 @@ def run_c2(args):
                             "fp32_partial_amplitude": amp_err32,
                             "slices_compared": n_ref},
         "other_split_modes": compare,
-        "refined_path": c2_refined_leg(args, ctx, net, label) if args.refined else None,
+        "other_plan": (c2_plan_leg(args, ctx, net, label, other) if args.other_plan
+                       else None),
+        "full_amplitude": (c2_full_amplitude(ctx, net, plan, batch, SPLIT_MODES[args.split][1])
+                           if args.full_amplitude else None),
         "fp64_mfma": {"ms_per_step": round(ms64, 3), "slices_per_step": n_ref},
         "kernels_split": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
                           for k, v in prof16.items()},
@@ -518,28 +547,58 @@ def run_c2(args):
                      "hw_16bit_mfma_frac": (round(achieved * SPLIT_MODES[args.split][3] / peak, 4)
                                             if dname == SPLIT_MODES[args.split][2] else None),
                      "algorithmic_bytes": round(dk["bytes"] / dk["launches"]),
-                     "traffic": profiled_traffic("c2_sliced_amplitude.md", dname),
-                     "traffic_source": "profiles/c2_sliced_amplitude.md"},
+                     "traffic": profiled_traffic(C2_PROFILE[args.c2_plan], dname),
+                     "traffic_source": "profiles/" + C2_PROFILE[args.c2_plan]},
     }, (plan, net)
 
 
-C2_REFINED_PLAN = os.path.join(ROOT, "jet_amd", "data", "grid_7x7x14_seed2026_s12_refined.json")
+def c2_full_amplitude(ctx, net, plan, batch, split_prec):
+    """The whole 4096-slice amplitude in the split mode, plain FP32 and
+    complex128 on FP64 MFMA (two fewer batched indices): its relative error
+    and the per-slice normwise error over every slice."""
+    from jet_amd.network import SlicedContraction
+    runs = {}
+    for name, prec, dtype, b in (("split", split_prec, np.complex64, batch),
+                                 ("fp32", "fp32", np.complex64, batch),
+                                 ("fp64", "fp64", np.complex128, batch[2:])):
+        data = [d.astype(dtype) for d in net.data]
+        sc = SlicedContraction(net.indices, net.shapes, data, plan["path"], plan["sliced"],
+                               batch=b, dtype=dtype, precision=prec, ctx=ctx, use_graph=False)
+        t0 = time.perf_counter()
+        chunk = max(1, sc.n_loop // 4)
+        for i in range(0, sc.n_loop, chunk):
+            sc.run(range(i, min(sc.n_loop, i + chunk)))
+            ctx.synchronize()
+            log(f"c2 full amplitude: {name} {min(sc.n_loop, i + chunk)}/{sc.n_loop} executions")
+        runs[name] = (sc.reduce(), sc.slice_results(), time.perf_counter() - t0)
+        sc.net.close()
+        del sc
+    a64, r64, _ = runs["fp64"]
+    out = {"slices": int(r64.size), "amplitude_fp64": [a64.real, a64.imag]}
+    for name in ("split", "fp32"):
+        a, r, sec = runs[name]
+        out[name] = {"amplitude_rel_err": abs(a - a64) / abs(a64),
+                     "per_slice_rel_err": float(np.linalg.norm(r - r64) / np.linalg.norm(r64)),
+                     "seconds": round(sec, 2)}
+    out["fp64"] = {"seconds": round(runs["fp64"][2], 2)}
+    log(f"c2 full amplitude: rel err split {out['split']['amplitude_rel_err']:.2e}, "
+        f"fp32 {out['fp32']['amplitude_rel_err']:.2e}")
+    return out
 
 
-def c2_refined_leg(args, ctx, net, label):
-    """The same amplitude along the refined path (the min-fill path after
-    jet_amd.paths.refine_sliced_path, `python -m jet_amd.paths --refine`):
-    about 4x fewer flops per slice, with more of its steps bound by memory
-    traffic than by the MFMA pipes.  Split and FP32 runs timed as one
-    captured graph each; errors against a complex128 FP64-MFMA run of the
-    first execution's first quarter of slices."""
-    import json
+def c2_plan_leg(args, ctx, net, label, plan_name):
+    """The same amplitude along the other C2 plan (min-fill, or refined for
+    batched execution), at that plan's default batch: split and FP32 runs
+    timed as one captured graph each (eager launches with --eager); errors
+    against a complex128 FP64-MFMA run of the first execution's first
+    quarter of slices; the split GEMM's own TFLOP/s from the event
+    profiler."""
     from jet_amd import _native as nat
     from jet_amd.network import SlicedContraction
-    with open(C2_REFINED_PLAN) as f:
-        plan = json.load(f)
-    path, sliced = [tuple(p) for p in plan["path"]], plan["sliced"]
-    batch = sliced[len(sliced) - args.batch_log2:] if args.batch_log2 else []
+    plan, _, _ = c2_problem(plan_name)
+    path, sliced = plan["path"], plan["sliced"]
+    nb = C2_BATCH_LOG2[plan_name]
+    batch = sliced[len(sliced) - nb:]
 
     def build(prec, dtype, b):
         data = [d.astype(dtype) for d in net.data]
@@ -547,8 +606,7 @@ def c2_refined_leg(args, ctx, net, label):
                                  dtype=dtype, precision=prec, ctx=ctx, use_graph=False)
 
     def timed(sc, iters):
-        """One warm-up execution, then `iters` timed ones: each a captured
-        graph (eager launches with --eager)."""
+        """One warm-up execution, then `iters` timed ones."""
         if args.eager:
             sc.run(range(1))
         else:
@@ -602,15 +660,20 @@ def c2_refined_leg(args, ctx, net, label):
     err = lambda r: float(np.linalg.norm(r[:n_ref] - res64) / np.linalg.norm(res64))
     sps16 = steps * n_batch / (ms16 * 1e-3)
     sps32 = n_batch / (ms32 * 1e-3)
-    log(f"c2 refined path: {label} {ms16 / steps:.1f} ms/execution ({sps16:.1f} slices/s, "
+    gemm = prof.get(SPLIT_MODES[args.split][2])
+    gemm_tf = gemm["flops"] / (gemm["ms"] * 1e-3) if gemm and gemm["ms"] > 0 else None
+    log(f"c2 {plan_name} path: {label} {ms16 / steps:.1f} ms/execution ({sps16:.1f} slices/s, "
         f"{exec_flops * steps / ms16 / 1e9:.1f} TFLOP/s), fp32 {ms32:.1f} ms/execution; "
         f"rel err per-slice {err(res16):.2e}, fp32 {err(res32):.2e}")
-    return {"plan": os.path.relpath(C2_REFINED_PLAN, ROOT),
+    return {"plan": os.path.relpath(C2_PLANS[plan_name], ROOT),
             "per_slice_pathinfo_flops": plan["per_slice_pathinfo_flops"],
             "per_slice_max_log2_size": plan["per_slice_max_log2_size"],
             "slices_per_step": n_batch, "ms_per_step": round(ms16 / steps, 3),
             "slices_per_s": round(sps16, 4),
             "tflops": round(exec_flops * steps / (ms16 * 1e-3) / 1e12, 3),
+            "split_gemm_tflops": round(gemm_tf / 1e12, 3) if gemm_tf else None,
+            "split_gemm_frac": round(gemm_tf / mfma_peak(SPLIT_MODES[args.split][2]), 4)
+                               if gemm_tf else None,
             "projected_amplitude_s": round(4096 / sps16, 1),
             "fp32_mfma": {"ms_per_step": round(ms32, 3), "slices_per_s": round(sps32, 4)},
             "speedup_split_vs_fp32": round(sps16 / sps32, 3),
@@ -618,22 +681,6 @@ def c2_refined_leg(args, ctx, net, label):
                                 "slices_compared": n_ref},
             "kernels_split": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
                               for k, v in prof.items()}}
-
-
-def run_c2r(args):
-    """The refined-path leg of c2 alone (its JSON line: slices/s of the
-    refined path); for profiling that path."""
-    from jet_amd import _native as nat
-    plan, circ, net = c2_problem()
-    r = c2_refined_leg(args, nat.default_context(), net, SPLIT_LABEL[args.split])
-    return {"metric": METRIC, "value": r["slices_per_s"], "unit": "slices/s",
-            "ms_per_step": r["ms_per_step"],
-            "dtype": f"c64 ({SPLIT_LABEL[args.split]} MFMA on MFMA-bound steps; fp32 MFMA "
-                     "and VALU elsewhere)",
-            "config": {"workload": "c2r: the c2 amplitude along the refined path; a step is "
-                                   f"one batched execution of {r['slices_per_step']} slices",
-                       "plan": r["plan"]},
-            "refined_path": r}, (plan, net)
 
 
 def c2_cpu_baseline(host, threads, budget_s=20.0):
@@ -675,12 +722,12 @@ def c2_cpu_baseline(host, threads, budget_s=20.0):
             break
     rate = done_flops / t_used
     per_slice = 4.0 * plan["per_slice_pathinfo_flops"]
-    return {"value": round(rate / 1e12, 4), "unit": "TFLOP/s", "cores": threads,
-            "kind": "port", "slices_per_s": rate / per_slice,
-            "sample": f"the {ran} most expensive contraction steps of one c2 slice that fit "
-                      f"{budget_s:.0f} s (QFlex transposes + OpenBLAS cblas_cgemm per step), "
-                      f"{done_flops:.3e} flops in {t_used:.1f} s; slices/s = that rate / the "
-                      f"per-slice flops"}
+    return {"value": round(rate / per_slice, 6), "unit": "slices/s", "cores": threads,
+            "kind": "port", "tflops": round(rate / 1e12, 4),
+            "sample": f"the {ran} most expensive contraction steps of one c2 slice (same plan) "
+                      f"that fit {budget_s:.0f} s (QFlex transposes + OpenBLAS cblas_cgemm per "
+                      f"step), {done_flops:.3e} flops in {t_used:.1f} s; slices/s = that rate / "
+                      f"the per-slice flops (8 M N K per step)"}
 
 
 C3_CIRCUITS = [((5, 5, 14), 24, 51), ((6, 6, 12), 16, 52), ((6, 7, 10), 24, 53)]
@@ -708,9 +755,10 @@ def run_c3(args):
         path = paths.best_minfill_path(net.indices, dims, trials=8, seed=1)
         bits = [tuple(int(x) for x in rng.integers(0, 2, r * c)) for _ in range(n_amp)]
         jobs.append(AmplitudeJob(circ, bits, path))
-    plan, circ2, net2 = c2_problem()
+    plan, circ2, net2 = c2_problem("min-fill")
     sliced = plan["sliced"]
-    batch = sliced[len(sliced) - args.batch_log2:] if args.batch_log2 else []
+    nb = C2_BATCH_LOG2["min-fill"] if args.batch_log2 is None else args.batch_log2
+    batch = sliced[len(sliced) - nb:] if nb else []
     per_exec = 2 ** len(batch)
     n_exec = max(1, args.c3_slices // per_exec)
     data32 = [d.astype(np.complex64) for d in net2.data]
@@ -845,13 +893,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2r", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--c2-plan", default="refined", choices=sorted(C2_PLANS),
+                    help="c2: contraction plan of the headline (the other one runs as a "
+                         "second leg unless --single-plan)")
     ap.add_argument("--c4-fd", type=int, default=0,
                     help="c4: angles checked by central differences (0: all)")
     ap.add_argument("--c3-slices", type=int, default=256,
                     help="c3: slices of the c2 amplitude in the batch")
-    ap.add_argument("--batch-log2", type=int, default=7,
-                    help="c2: sliced indices batched per execution (2^b slices per step)")
+    ap.add_argument("--batch-log2", type=int, default=None,
+                    help="c2: sliced indices batched per execution (2^b slices per step; "
+                         "default 8 on the refined plan, 7 on the min-fill one)")
     ap.add_argument("--split", default="f16", choices=sorted(SPLIT_MODES),
                     help="split 16-bit MFMA GEMM: f16 (split-FP16 4M), 4m (split-BF16 "
                          "4M), 3m (split-BF16 3M)")
@@ -860,8 +912,11 @@ def main():
     ap.add_argument("--steps32", type=int, default=1,
                     help="c2: executions timed in plain FP32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-refined", dest="refined", action="store_false",
-                    help="c2: skip the refined-path leg")
+    ap.add_argument("--single-plan", dest="other_plan", action="store_false",
+                    help="c2: skip the other plan's leg")
+    ap.add_argument("--full-amplitude", action="store_true",
+                    help="c2: also contract all 4096 slices in the split mode, FP32 and "
+                         "FP64 (minutes) and report the amplitude's errors")
     ap.add_argument("--eager", action="store_true",
                     help="c2: launch the timed executions eagerly instead of as one "
                          "captured hipGraph (for kernel tracers)")
@@ -885,9 +940,6 @@ def main():
         args.no_cpu_baseline = True
     elif args.workload == "c4":
         result, host = run_c4(args)
-        args.no_cpu_baseline = True
-    elif args.workload == "c2r":
-        result, host = run_c2r(args)
         args.no_cpu_baseline = True
     else:
         result, host = run_c2(args)

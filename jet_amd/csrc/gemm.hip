@@ -518,45 +518,70 @@ __global__ __launch_bounds__(256) void split_f16_kernel(
     }
 }
 
-// Rows of at most kSplitOnePassK padded values: one wave per row holds the
-// whole row in registers, takes its maximum with a wave reduction and
-// writes the planes and the scale in the same pass (the operand is read
-// once instead of twice).  Same planes and scales as the two-pass path.
+// Rows of at most kSplitOnePassK padded values: a group of G lanes (G a
+// power of two, 4 <= G <= 64, the smallest with G >= k_p / 8) holds one row
+// in registers, eight consecutive values per lane and chunk, takes its
+// maximum with a shuffle reduction inside the group and writes the planes
+// and the scale in the same pass (the operand is read once instead of
+// twice).  64 / G rows per wave, so short rows (K = 128: 16 lanes) keep
+// every lane busy.  `vec` (host-checked: 16-byte aligned source, even
+// leading dimension and batch stride, K a multiple of 8) loads the eight
+// values as four 16-byte loads.  Same planes and scales as the two-pass
+// path.
 constexpr int kSplitOnePassK = 2048;
 
 __global__ __launch_bounds__(256) void split_f16_rows_kernel(
     const float2 *__restrict__ src, int64_t rows, int64_t K, int64_t ld,
-    int64_t s, SplitOperand dst)
+    int64_t s, SplitOperand dst, int log2_group, int vec)
 {
-    constexpr int CH = kSplitOnePassK / 8 / 64; // 8-value chunks per lane
+    constexpr int CH = kSplitOnePassK / 8 / 64; // 8-value chunks per lane at G = 64
     const int lane = threadIdx.x & 63;
-    const int64_t row = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    if (row >= dst.batch * dst.rows_p) return;
-    const int64_t b = row / dst.rows_p, r = row % dst.rows_p;
+    const int G = 1 << log2_group;
+    const int gl = lane & (G - 1);
+    const int64_t row = (int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * (64 >> log2_group) +
+                        (lane >> log2_group);
+    const int64_t nrow = dst.batch * dst.rows_p;
+    const bool live = row < nrow;
+    const int64_t b = live ? row / dst.rows_p : 0, r = live ? row % dst.rows_p : 0;
+    const bool in_row = live && r < rows;
     const int kq_n = int(dst.k_p / 8);
+    const float2 *rp = src + b * s + r * ld;
     float2 x[CH][8];
     uint32_t m = 0u;
 #pragma unroll
     for (int c = 0; c < CH; c++) {
-        const int kq = c * 64 + lane;
+        const int kq = c * G + gl;
+        const bool any = in_row && kq < kq_n && int64_t(kq) * 8 < K;
+        if (vec && any) {
+            const float4 *q = reinterpret_cast<const float4 *>(rp + int64_t(kq) * 8);
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int64_t k = int64_t(kq) * 8 + e;
-            x[c][e] = (kq < kq_n && r < rows && k < K) ? src[b * s + r * ld + k]
-                                                       : float2{0.f, 0.f};
+            for (int e = 0; e < 4; e++) {
+                const float4 v = q[e];
+                x[c][2 * e] = float2{v.x, v.y};
+                x[c][2 * e + 1] = float2{v.z, v.w};
+            }
+        }
+        else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int64_t k = int64_t(kq) * 8 + e;
+                x[c][e] = (any && k < K) ? rp[k] : float2{0.f, 0.f};
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++)
             m = max(m, max(__float_as_uint(x[c][e].x) & 0x7fffffffu,
                            __float_as_uint(x[c][e].y) & 0x7fffffffu));
-        }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+    for (int off = G >> 1; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+    if (!live) return;
     const int sh = r < rows ? row_shift(m) : 0;
     const float sc = __builtin_ldexpf(1.f, sh);
-    if (lane == 0) dst.scale[row] = __builtin_ldexpf(1.f, -sh);
+    if (gl == 0) dst.scale[row] = __builtin_ldexpf(1.f, -sh);
     _Float16 *base = static_cast<_Float16 *>(dst.base);
 #pragma unroll
     for (int c = 0; c < CH; c++) {
-        const int kq = c * 64 + lane;
+        const int kq = c * G + gl;
         if (kq >= kq_n) continue;
         f16x8 o[4];
 #pragma unroll
@@ -1343,9 +1368,15 @@ void launch_split_f16(const void *src, int64_t rows, int64_t k, int64_t ld,
         prof::Scope ps(prof::kSplitF16, stream, 0.0,
                        8.0 * double(batch) * double(rows) * double(k) +
                            8.0 * double(nrow) * double(dst.k_p));
-        JH_REQUIRE(ceil_div(nrow, 4) <= 0x7fffffff, "Too many rows.");
-        hipLaunchKernelGGL(split_f16_rows_kernel, dim3((unsigned)ceil_div(nrow, 4)), dim3(256), 0,
-                           stream, static_cast<const float2 *>(src), rows, k, ld, s, dst);
+        int lg = 2;
+        while ((1 << lg) < 64 && (int64_t(1) << lg) * 8 < dst.k_p) lg++;
+        const int64_t rows_per_block = 4 * (64 >> lg);
+        const int vec = (reinterpret_cast<uintptr_t>(src) % 16 == 0 && ld % 2 == 0 &&
+                         s % 2 == 0 && k % 8 == 0) ? 1 : 0;
+        JH_REQUIRE(ceil_div(nrow, rows_per_block) <= 0x7fffffff, "Too many rows.");
+        hipLaunchKernelGGL(split_f16_rows_kernel, dim3((unsigned)ceil_div(nrow, rows_per_block)),
+                           dim3(256), 0, stream, static_cast<const float2 *>(src), rows, k, ld, s,
+                           dst, lg, vec);
         JH_HIP(hipGetLastError());
         return;
     }

@@ -716,20 +716,53 @@ def mi355x_step_seconds(log2_m: float, log2_n: float, log2_k: float,
     return t * 2.0 ** log2_batch + launch_s / 2.0 ** batch_bits
 
 
-# Two refinement stages: output-light model first, then the full operand
-# preparation cost (the second stage alone stops in a worse local optimum).
-REFINE_STAGES = ((12.0, 0.0, 7.0), (40.0, 16.0, 6.0))
+# Refinement stages, as (operand_bytes, output_bytes) of mi355x_step_seconds:
+# an output-light model first, then the full operand preparation cost (the
+# second stage alone stops in a worse local optimum).
+REFINE_STAGES = ((12.0, 0.0), (40.0, 16.0))
 
 
-def refine_sliced_path(indices, dims, path: Path, sliced: Sequence[str], max_log2_size: float,
-                       subtree_size: int = 10, stages=REFINE_STAGES) -> Path:
-    """Subtree reconfiguration of a sliced path against mi355x_step_seconds,
-    one pass set per (operand_bytes, output_bytes, batch_bits) stage."""
-    for ob, cb, bb in stages:
-        def cost(lm, ln, lk, ob=ob, cb=cb, bb=bb):
-            return mi355x_step_seconds(lm, ln, lk, ob, cb, batch_bits=bb)
-        path = reconfigure_path(indices, dims, path, sliced, max_log2_size=max_log2_size,
-                                subtree_size=subtree_size, passes=20, step_cost=cost)
+def batched_tree(indices, dims, path: Path, sliced: Sequence[str], batch_log2: int,
+                 operand_bytes: float = 40.0, output_bytes: float = 16.0) -> "_Tree":
+    """The contraction tree of one batched execution: the last `batch_log2`
+    sliced indices stay open (batch modes), the others are fixed; each
+    step's cost is mi355x_step_seconds of its batched GEMM (one launch per
+    execution), so total() models the seconds of one execution."""
+    nb = int(batch_log2)
+    loop, batch = list(sliced[:len(sliced) - nb]), list(sliced[len(sliced) - nb:])
+
+    def cost(lm, ln, lk, lb):
+        return mi355x_step_seconds(lm, ln, lk, operand_bytes, output_bytes, batch_bits=0.0,
+                                   log2_batch=lb)
+    return _Tree(indices, dims, path, loop, step_cost=cost, open_labels=batch)
+
+
+def refine_batched_path(indices, dims, path: Path, sliced: Sequence[str], batch_log2: int,
+                        extra_log2: float = 1.0, subtree_size: int = 10, rounds: int = 2,
+                        stages=REFINE_STAGES) -> Path:
+    """Subtree reconfiguration of a sliced path for batched execution: the
+    last `batch_log2` sliced indices are batch modes of every step that
+    holds them (open labels), the others are sliced away, and each step
+    costs the modelled seconds of its batched launch (batched_tree).  The
+    largest batched intermediate may grow by `extra_log2` over the input
+    path's.  Each round runs the stages in order; rounds stop early when one
+    does not lower the modelled execution time."""
+    nb = int(batch_log2)
+    loop, batch = list(sliced[:len(sliced) - nb]), list(sliced[len(sliced) - nb:])
+    limit = batched_tree(indices, dims, path, sliced, nb).max_log2size() + extra_log2
+    best = batched_tree(indices, dims, path, sliced, nb).total()
+    for _ in range(rounds):
+        cand = path
+        for ob, cb in stages:
+            def cost(lm, ln, lk, lb, ob=ob, cb=cb):
+                return mi355x_step_seconds(lm, ln, lk, ob, cb, batch_bits=0.0, log2_batch=lb)
+            cand = reconfigure_path(indices, dims, cand, loop, max_log2_size=limit,
+                                    subtree_size=subtree_size, passes=20, step_cost=cost,
+                                    open_labels=batch)
+        t = batched_tree(indices, dims, cand, sliced, nb).total()
+        if t >= best:
+            break
+        path, best = cand, t
     return path
 
 
@@ -814,31 +847,38 @@ if __name__ == "__main__":
     ap.add_argument("--max-log2-size", type=float, default=29.0)
     ap.add_argument("--workers", type=int, default=1)
     ap.add_argument("--refine", metavar="PLAN_JSON",
-                    help="refine an existing plan's path (refine_sliced_path) and order its "
-                         "sliced indices so that the last ones batch within --batch-gib")
-    ap.add_argument("--batch-gib", type=float, default=150.0)
+                    help="refine an existing plan's path for batched execution "
+                         "(refine_batched_path) and write it to --out")
+    ap.add_argument("--batch-log2", type=int, default=7,
+                    help="--refine: sliced indices kept as batch modes (the last ones)")
+    ap.add_argument("--extra-log2", type=float, default=1.0,
+                    help="--refine: growth allowed for the largest batched intermediate")
+    ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--out", help="where --refine writes (default: over its input)")
     a = ap.parse_args()
     if a.refine:
         from . import circuits
-        from .network import choose_batch
         with open(a.refine) as f:
             plan = json.load(f)
         circ = circuits.random_grid_circuit(plan["rows"], plan["cols"], plan["depth"],
                                             seed=plan["seed"])
         net = circuits.amplitude_network(circ)
         dims = {l: 2 for ix in net.indices for l in ix}
-        mx = plan["per_slice_max_log2_size"]
-        path = refine_sliced_path(net.indices, dims, [tuple(p) for p in plan["path"]],
-                                  plan["sliced"], mx)
-        batch = choose_batch(net.indices, net.shapes, path, plan["sliced"], a.batch_gib * 2**30)
-        sliced = [l for l in plan["sliced"] if l not in batch] + batch
+        path0 = [tuple(p) for p in plan["path"]]
+        sliced = plan["sliced"]
+        path = refine_batched_path(net.indices, dims, path0, sliced, a.batch_log2, a.extra_log2,
+                                   rounds=a.rounds)
+        t0 = batched_tree(net.indices, dims, path0, sliced, a.batch_log2).total()
+        t1 = batched_tree(net.indices, dims, path, sliced, a.batch_log2).total()
         full = path_stats(net.indices, dims, path)
         per = path_stats(net.indices, dims, path, sliced=sliced)
-        plan.update({"path": [list(p) for p in path], "sliced": sliced,
-                     "refined": {"method": "reconfigure_path against mi355x_step_seconds",
+        plan.update({"path": [list(p) for p in path],
+                     "refined": {"method": "refine_batched_path (reconfigure_path with the "
+                                           "batched indices open, against mi355x_step_seconds)",
                                  "stages": [list(x) for x in REFINE_STAGES], "subtree_size": 10,
-                                 "batched_indices": len(batch), "batch_budget_gib": a.batch_gib,
+                                 "rounds": a.rounds, "batch_log2": a.batch_log2,
+                                 "extra_log2": a.extra_log2,
+                                 "modelled_execution_s": [t0, t1],
                                  "unrefined_per_slice_pathinfo_flops":
                                      plan["per_slice_pathinfo_flops"]},
                      "pathinfo_flops": full.total_flops, "max_log2_size": full.max_log2_size,
@@ -847,8 +887,8 @@ if __name__ == "__main__":
         out = a.out or a.refine
         with open(out, "w") as f:
             json.dump(plan, f)
-        print(out, "per slice %.3e max %d, %d batched indices" % (
-            per.total_flops, per.max_log2_size, len(batch)))
+        print(out, "per slice %.3e max %d; modelled execution %.1f -> %.1f ms" % (
+            per.total_flops, per.max_log2_size, t0 * 1e3, t1 * 1e3))
         raise SystemExit(0)
     plan = grid_circuit_plan(a.rows, a.cols, a.depth, a.seed, a.slices, a.trials, a.max_log2_size,
                              a.workers)

@@ -394,3 +394,27 @@ def test_split_f16_row_scales(ctx):
     norm = np.abs(a).max(1)[:, None] * np.abs(b).max(1)[None, :] * np.sqrt(K)
     err = np.abs(got.reshape(M, N) - want) / norm
     assert err.max() < 1e-6, err.max()
+
+
+@pytest.mark.parametrize("M,N,K", [(333, 150, 128), (200, 131, 130), (129, 260, 256),
+                                   (150, 140, 300), (140, 300, 520), (130, 129, 2048),
+                                   (131, 140, 2050)])
+def test_split_f16_one_pass_groups(ctx, M, N, K):
+    """The one-pass split (rows of at most 2048 padded values) packs several
+    short rows into one wave (16 lanes per row at K = 128, 32 at K <= 256)
+    and uses 16-byte loads when K is a multiple of 8; longer rows take the
+    two-pass path.  Neighbouring rows of very different magnitudes must each
+    get their own scale: error per output against |A row| |B row| sqrt(K)."""
+    rng = np.random.default_rng(M + 3 * N + 7 * K)
+    sa = 10.0 ** rng.uniform(-8, 8, M)
+    sb = 10.0 ** rng.uniform(-8, 8, N)
+    a = (rand_c(rng, M * K, np.complex128).reshape(M, K) * sa[:, None]).astype(np.complex64)
+    b = (rand_c(rng, N * K, np.complex128).reshape(N, K) * sb[:, None]).astype(np.complex64)
+    say(f"split-f16 one-pass groups M={M} N={N} K={K}")
+    got, info = contract(ctx, a.reshape(-1), [0, 1], [M, K], b.reshape(-1), [2, 1], [N, K],
+                         [0, 2], precision=nat.PREC_F16X3_4M)
+    assert info["kernel"] == "f16x3_4m_mfma", info
+    want = a.astype(np.complex128) @ b.astype(np.complex128).T
+    norm = np.abs(a).max(1)[:, None] * np.abs(b).max(1)[None, :] * np.sqrt(K)
+    err = np.abs(got.reshape(M, N) - want) / norm
+    assert err.max() < 1e-6, err.max()

@@ -122,9 +122,12 @@ def test_step_time_model():
 
 
 def test_committed_c2_plans_are_consistent():
-    """The committed C2 plans (min-fill, and refined) are valid SSA paths of
-    the seeded 7x7 depth-14 amplitude network; their recorded per-slice
-    PathInfo flops and sizes match path_stats; the refined one is cheaper."""
+    """The committed C2 plans (min-fill, and refined for batched execution)
+    are valid SSA paths of the seeded 7x7 depth-14 amplitude network over
+    the same sliced indices; their recorded per-slice PathInfo flops and
+    sizes match path_stats; the refined one is cheaper per slice and in the
+    modelled batched execution it was refined for, whose largest batched
+    intermediate grew by at most the allowed extra_log2."""
     import json
     import os
     root = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
@@ -132,7 +135,7 @@ def test_committed_c2_plans_are_consistent():
     circ = circuits.random_grid_circuit(7, 7, 14, seed=2026)
     net = circuits.amplitude_network(circ)
     dims = {l: 2 for ix in net.indices for l in ix}
-    per = {}
+    per, plans = {}, {}
     for name in ("grid_7x7x14_seed2026_s12.json", "grid_7x7x14_seed2026_s12_refined.json"):
         with open(os.path.join(root, name)) as f:
             plan = json.load(f)
@@ -143,4 +146,15 @@ def test_committed_c2_plans_are_consistent():
         assert st.total_flops == plan["per_slice_pathinfo_flops"]
         assert st.max_log2_size == plan["per_slice_max_log2_size"] <= 28
         per[name] = st.total_flops
+        plans[name] = (path, plan)
+    base_path, base = plans["grid_7x7x14_seed2026_s12.json"]
+    ref_path, ref = plans["grid_7x7x14_seed2026_s12_refined.json"]
+    assert ref["sliced"] == base["sliced"]
     assert per["grid_7x7x14_seed2026_s12_refined.json"] < per["grid_7x7x14_seed2026_s12.json"] / 3
+    info = ref["refined"]
+    nb = info["batch_log2"]
+    t0 = paths.batched_tree(net.indices, dims, base_path, base["sliced"], nb)
+    t1 = paths.batched_tree(net.indices, dims, ref_path, ref["sliced"], nb)
+    assert [t0.total(), t1.total()] == pytest.approx(info["modelled_execution_s"], rel=1e-12)
+    assert t1.total() < 0.5 * t0.total()
+    assert t1.max_log2size() <= t0.max_log2size() + info["extra_log2"]
