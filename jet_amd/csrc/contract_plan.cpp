@@ -36,11 +36,15 @@ int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
     const bool mfma_shape = M >= 32 && N >= 16 && K >= 16;
     if (mfma_shape) {
         if (mode == 2) return kGemmF64Mfma;
-        // Split-BF16 pays only where the step is bound by MFMA throughput
-        // (every dimension large enough to reuse operands from LDS).
+        // The split modes pay where the step is bound by MFMA throughput
+        // (every dimension large enough to reuse operands from LDS).  The
+        // split-FP16 kernel also beats FP32 MFMA from K = 64 (1.4-2.1x on
+        // 256 x 2^23 x 64 .. 16384^2 x 64), and at K = 32 on large outputs.
         if (mode == 1 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x3;
         if (mode == 3 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x4m;
-        if (mode == 4 && M >= 128 && N >= 128 && K >= 128) return kGemmF16x4m;
+        if (mode == 4 && M >= 128 && N >= 128 &&
+            (K >= 64 || (K >= 32 && batch * M * N >= (int64_t(1) << 24))))
+            return kGemmF16x4m;
         return kGemmF32Mfma;
     }
     if (K >= 4 * kSplitKChunk && batch * M * N <= 65536) return kGemmSplitK;

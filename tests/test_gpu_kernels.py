@@ -165,6 +165,8 @@ def test_contract_random_small(ctx, prec, dtype):
     (nat.PREC_F16X3_4M, np.complex64, (300, 200, 170)),
     (nat.PREC_F16X3_4M, np.complex64, (1000, 260, 1100)),
     (nat.PREC_F16X3_4M, np.complex64, (300, 4100, 200)),   # 128x192 tiles, ragged edges
+    (nat.PREC_F16X3_4M, np.complex64, (300, 260, 64)),     # short K on the split kernel
+    (nat.PREC_F16X3_4M, np.complex64, (4096, 4100, 32)),
     (nat.PREC_BF16X3_4M, np.complex64, (130, 4500, 160)),
     (nat.PREC_FP64, np.complex128, (256, 128, 96)),
     (nat.PREC_FP64, np.complex128, (130, 70, 50)),
@@ -196,7 +198,8 @@ def test_gemm_kernels(ctx, prec, dtype, shape):
         assert info["kernel"] == "bf16x3_mfma"
     if prec == nat.PREC_BF16X3_4M and min(shape) >= 128:
         assert info["kernel"] == "bf16x3_4m_mfma"
-    if prec == nat.PREC_F16X3_4M and min(shape) >= 128:
+    if prec == nat.PREC_F16X3_4M and min(shape[:2]) >= 128 and (
+            shape[2] >= 64 or (shape[2] >= 32 and shape[0] * shape[1] >= 1 << 24)):
         assert info["kernel"] == "f16x3_4m_mfma"
 
 
