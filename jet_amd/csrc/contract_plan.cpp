@@ -30,6 +30,90 @@ PermutePlan plan_reorder(const std::vector<int32_t> &from,
 
 } // namespace
 
+bool plan_split_gather(const std::vector<int32_t> &src, const std::vector<int64_t> &ext,
+                       const std::vector<int32_t> &target, int64_t K, SplitGather &g)
+{
+    // log2 extents; every extent a power of two
+    const int r = (int)src.size();
+    std::vector<int> lg(r);
+    for (int i = 0; i < r; i++) {
+        if (ext[i] < 1 || (ext[i] & (ext[i] - 1))) return false;
+        int b = 0;
+        while ((int64_t(1) << b) < ext[i]) b++;
+        lg[i] = b;
+    }
+    // source bit offset of each mode (row-major: the last mode is fastest)
+    std::vector<int> soff(r);
+    int nbits = 0;
+    for (int i = r - 1; i >= 0; i--) {
+        soff[i] = nbits;
+        nbits += lg[i];
+    }
+    if (nbits > 62 || K < 32 || K > 1024 || (K & (K - 1))) return false;
+    int log2k = 0;
+    while ((int64_t(1) << log2k) < K) log2k++;
+    // source bit of every target bit
+    std::vector<int> srcbit(nbits), tgt_of_src(nbits);
+    int toff = 0;
+    for (int i = (int)target.size() - 1; i >= 0; i--) {
+        const int m = find(src, target[i]);
+        if (m < 0) return false;
+        for (int q = 0; q < lg[m]; q++) srcbit[toff + q] = soff[m] + q;
+        toff += lg[m];
+    }
+    if (toff != nbits) return false;
+    for (int t = 0; t < nbits; t++) tgt_of_src[srcbit[t]] = t;
+    const int nrow = nbits - log2k;
+    if (nrow < 4) return false;
+    // Tile row bits: those holding the lowest source bits, until the tile
+    // (all K bits + these) reaches kGatherTileBits; then the lowest other
+    // row bits.
+    std::vector<char> chosen(nrow, 0);
+    int budget = kGatherTileBits - log2k, covered = 0;
+    for (int sb = 0; sb < nbits; sb++) {
+        const int t = tgt_of_src[sb];
+        if (t >= log2k) {
+            if (budget == 0) break;
+            chosen[t - log2k] = 1;
+            budget--;
+        }
+        covered = sb + 1;
+    }
+    if (covered < 4) return false;
+    for (int rr = 0; rr < nrow && budget > 0; rr++)
+        if (!chosen[rr]) {
+            chosen[rr] = 1;
+            budget--;
+        }
+    g = SplitGather{};
+    g.log2k = log2k;
+    std::vector<int> vbit_of_t(nbits, -1);
+    for (int t = 0; t < log2k; t++) vbit_of_t[t] = t;
+    for (int rr = 0; rr < nrow; rr++)
+        if (chosen[rr]) {
+            g.vrow[g.xr] = (int8_t)rr;
+            vbit_of_t[log2k + rr] = log2k + g.xr;
+            g.xr++;
+        }
+    g.tb = log2k + g.xr;
+    // u order: the tile's source bits ascending
+    int j = 0;
+    for (int sb = 0; sb < nbits; sb++) {
+        const int t = tgt_of_src[sb];
+        if (vbit_of_t[t] >= 0) {
+            g.su[j] = (int8_t)sb;
+            g.u2v[j] = (int8_t)vbit_of_t[t];
+            j++;
+        }
+        else {
+            g.wsrc[g.nw] = (int8_t)sb;
+            g.wrow[g.nw] = (int8_t)(t - log2k);
+            g.nw++;
+        }
+    }
+    return j == g.tb;
+}
+
 int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
                        int64_t batch)
 {
@@ -142,6 +226,7 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
     // leaves fewer elements to permute.
     PermutePlan best_a, best_b;
     int64_t best_cost = -1;
+    int best_opt = 0;
     for (int opt = 0; opt < 2; opt++) {
         const auto &common = opt == 0 ? common_a : common_b;
         PermutePlan pa = plan_reorder(A, EA, cat(batch, left, common));
@@ -152,6 +237,7 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
             best_cost = cost;
             best_a = pa;
             best_b = pb;
+            best_opt = opt;
         }
     }
     d.pa = best_a;
@@ -168,14 +254,23 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
 
     d.gemm_kernel = choose_gemm_kernel(mode, d.M, d.N, d.K, d.batch);
     d.flops = 8.0 * double(d.batch) * double(d.M) * double(d.N) * double(d.K);
+    if (d.gemm_kernel == kGemmF16x4m) {
+        // fold a permutation into the split pass when the rows need no
+        // padding (the split rows are then the permuted rows)
+        const auto &common = best_opt == 0 ? common_a : common_b;
+        if (d.need_a && d.M % kSplitRowPad == 0)
+            d.fuse_a = plan_split_gather(A, EA, cat(batch, left, common), d.K, d.ga);
+        if (d.need_b && d.N % kSplitRowPad == 0)
+            d.fuse_b = plan_split_gather(B, EB, cat(batch, right, common), d.K, d.gb);
+    }
 
     size_t off = 0;
     const size_t eb = (size_t)elem_bytes;
-    if (d.need_a) {
+    if (d.need_a && !d.fuse_a) {
         d.off_at = off;
         off = align256(off + eb * (size_t)d.size_a);
     }
-    if (d.need_b) {
+    if (d.need_b && !d.fuse_b) {
         d.off_bt = off;
         off = align256(off + eb * (size_t)d.size_b);
     }

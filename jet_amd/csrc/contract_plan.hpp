@@ -21,6 +21,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace jet_hip {
 
@@ -30,6 +31,9 @@ struct ContractionDesc {
     int64_t batch = 1, M = 1, N = 1, K = 1;
     int64_t size_a = 1, size_b = 1, size_c = 1;
     bool need_a = false, need_b = false, need_c = false;
+    // split-FP16: the permutation of A (B) folded into its split pass
+    bool fuse_a = false, fuse_b = false;
+    SplitGather ga, gb;
     PermutePlan pa, pb, pc;
     int gemm_kernel = kGemmSmall;
     // Workspace layout (bytes, 256-aligned offsets).
@@ -47,5 +51,14 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
                                  const int64_t *ext_a, int rb,
                                  const int32_t *modes_b, const int64_t *ext_b,
                                  int rc, const int32_t *modes_c);
+
+// Gather plan for the split-FP16 operand [batch][rows][K] (target, K the
+// product of the last modes' extents) of a source tensor (row-major modes
+// and extents); false when the folded pass does not apply: an extent that
+// is not a power of two, K outside 32..1024, fewer than 16 rows, or a
+// source whose four lowest address bits (a 128-byte line) do not fit in a
+// tile with the K bits.
+bool plan_split_gather(const std::vector<int32_t> &src, const std::vector<int64_t> &ext,
+                       const std::vector<int32_t> &target, int64_t K, SplitGather &g);
 
 } // namespace jet_hip

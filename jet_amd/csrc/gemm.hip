@@ -600,6 +600,154 @@ __global__ __launch_bounds__(256) void split_f16_rows_kernel(
     }
 }
 
+// Split-FP16 planes gathered from an unpermuted source (SplitGather): one
+// work-group per tile of 2^tb elements (all K values of 2^xr rows).
+//   1. the tile is read in source order, two neighbouring elements per
+//      16-byte load (source bit 0 is always a tile bit), into LDS in source
+//      order, XOR-swizzled (gather_lds_index);
+//   2. each thread takes its 8-value chunks of target rows from LDS into
+//      registers and folds them into the row maxima (LDS atomics);
+//   3. the planes and scales are written from registers as
+//      split_f16_rows_kernel writes them.
+// Every per-element offset is computed once per thread, outside the tile
+// loop.
+// LDS slot of tile element u: its low five bits (the bank of an 8-byte
+// element in a 256-byte bank row) XORed with bits 5-9 and 10-11, so that the
+// lanes of the row phase, which step through arbitrary u bits, spread over
+// the banks.
+__device__ __forceinline__ int gather_lds_index(int u)
+{
+    return u ^ (((u >> 5) ^ (u >> 10)) & 31);
+}
+
+constexpr int kGatherPairs = (1 << kGatherTileBits) / 2 / 256;
+constexpr int kGatherChunks = (1 << kGatherTileBits) / 8 / 256;
+
+__global__ __launch_bounds__(256) void split_f16_gather_kernel(
+    const float2 *__restrict__ src, SplitGather g, SplitOperand dst, int vec)
+{
+    __shared__ float2 tile[1 << kGatherTileBits];
+    __shared__ uint32_t rowmax[128];
+    const int tid = threadIdx.x;
+    const int tsz = 1 << g.tb;
+    const int kq_bits = g.log2k - 3;
+    const int nrows = 1 << g.xr;
+    // v-bit -> u-bit
+    int v2u[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) v2u[j] = 0;
+    for (int j = 0; j < g.tb; j++) v2u[g.u2v[j]] = j;
+    // loads: pairs (u, u + 1), u even
+    int64_t soff[kGatherPairs];
+    int ls0[kGatherPairs], ls1[kGatherPairs];
+#pragma unroll
+    for (int e = 0; e < kGatherPairs; e++) {
+        const int u = 2 * (tid + e * 256);
+        int64_t o = 0;
+        for (int j = 1; j < g.tb; j++)
+            if ((u >> j) & 1) o += int64_t(1) << g.su[j];
+        soff[e] = o;
+        ls0[e] = gather_lds_index(u);
+        ls1[e] = gather_lds_index(u + 1);
+    }
+    // chunks: 8 consecutive k of one target row
+    int ue[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        int u = 0;
+        for (int j = 0; j < 3; j++)
+            if ((e >> j) & 1) u |= 1 << v2u[j];
+        ue[e] = u;
+    }
+    int cbase[kGatherChunks], crow[kGatherChunks], ckq[kGatherChunks];
+    int64_t croff[kGatherChunks];
+#pragma unroll
+    for (int c = 0; c < kGatherChunks; c++) {
+        const int q = tid + c * 256;
+        const int v = q * 8;
+        int u = 0;
+        for (int j = 3; j < g.tb; j++)
+            if ((v >> j) & 1) u |= 1 << v2u[j];
+        cbase[c] = u;
+        crow[c] = q >> kq_bits;
+        ckq[c] = q & ((1 << kq_bits) - 1);
+        int64_t ro = 0;
+        for (int i = 0; i < g.xr; i++)
+            if ((crow[c] >> i) & 1) ro += int64_t(1) << g.vrow[i];
+        croff[c] = ro;
+    }
+    const int64_t ntiles = int64_t(1) << g.nw;
+    const int kt = 1 << (g.log2k - 5); // 32-wide k tiles per row
+    _Float16 *base = static_cast<_Float16 *>(dst.base);
+    for (int64_t w = blockIdx.x; w < ntiles; w += gridDim.x) {
+        int64_t sbase = 0, rbase = 0;
+        for (int j = 0; j < g.nw; j++)
+            if ((w >> j) & 1) {
+                sbase += int64_t(1) << g.wsrc[j];
+                rbase += int64_t(1) << g.wrow[j];
+            }
+        if (tid < nrows) rowmax[tid] = 0u;
+#pragma unroll
+        for (int e = 0; e < kGatherPairs; e++)
+            if (2 * (tid + e * 256) < tsz) {
+                const float2 *q = src + sbase + soff[e];
+                if (vec) {
+                    const float4 x = *reinterpret_cast<const float4 *>(q);
+                    tile[ls0[e]] = float2{x.x, x.y};
+                    tile[ls1[e]] = float2{x.z, x.w};
+                }
+                else {
+                    tile[ls0[e]] = q[0];
+                    tile[ls1[e]] = q[1];
+                }
+            }
+        __syncthreads();
+        float2 x[kGatherChunks][8];
+#pragma unroll
+        for (int c = 0; c < kGatherChunks; c++) {
+            uint32_t m = 0u;
+            const bool live = 8 * (tid + c * 256) < tsz;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                x[c][e] = live ? tile[gather_lds_index(cbase[c] | ue[e])] : float2{0.f, 0.f};
+                m = max(m, max(__float_as_uint(x[c][e].x) & 0x7fffffffu,
+                               __float_as_uint(x[c][e].y) & 0x7fffffffu));
+            }
+            // lanes of one row are consecutive: reduce within each group of
+            // min(64, chunks per row) lanes, then one LDS atomic per group
+            const int grp = kq_bits < 6 ? (1 << kq_bits) : 64;
+            for (int off = grp >> 1; off > 0; off >>= 1)
+                m = max(m, (uint32_t)__shfl_xor((int)m, off));
+            if (live && (tid & (grp - 1)) == 0) atomicMax(&rowmax[crow[c]], m);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < kGatherChunks; c++) {
+            if (8 * (tid + c * 256) >= tsz) continue;
+            const int sh = row_shift(rowmax[crow[c]]);
+            const float sc = __builtin_ldexpf(1.f, sh);
+            const int64_t row = rbase + croff[c];
+            const int kq = ckq[c];
+            if (kq == 0) dst.scale[row] = __builtin_ldexpf(1.f, -sh);
+            f16x8 o[4];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                _Float16 h, l;
+                split2h(x[c][e].x * sc, h, l);
+                o[0][e] = h;
+                o[1][e] = l;
+                split2h(x[c][e].y * sc, h, l);
+                o[2][e] = h;
+                o[3][e] = l;
+            }
+            const int64_t off = (row * kt + kq / 4) * 128 + (kq % 4) * 8;
+#pragma unroll
+            for (int p = 0; p < 4; p++) *reinterpret_cast<f16x8 *>(base + off + p * 32) = o[p];
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 3M split-BF16 complex GEMM.
 // Stage s covers product s/nk (T1, T2, T3) and k tile s%nk: it loads the hi
@@ -1393,6 +1541,25 @@ void launch_split_f16(const void *src, int64_t rows, int64_t k, int64_t ld,
                            batch, dst.rows_p, dst.rowmax);
     hipLaunchKernelGGL(split_f16_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)),
                        dim3(256), 0, stream, static_cast<const float2 *>(src), rows, k, ld, s, dst);
+    JH_HIP(hipGetLastError());
+}
+
+void launch_split_f16_gather(const void *src, const SplitGather &g, const SplitOperand &dst,
+                             hipStream_t stream)
+{
+    JH_REQUIRE(dst.planes == 4 && dst.interleaved && dst.scale &&
+                   dst.k_p == (int64_t(1) << g.log2k) && g.tb <= kGatherTileBits &&
+                   g.tb >= 8 && g.xr >= 1 && g.xr <= 7 && g.log2k >= 5 &&
+                   dst.batch * dst.rows_p == (int64_t(1) << (g.xr + g.nw)),
+               "Gathered split operand does not match its plan.");
+    const int64_t n = int64_t(1) << (g.tb + g.nw);
+    // reads the source once, writes 4 fp16 per element
+    prof::Scope ps(prof::kSplitF16, stream, 0.0, 16.0 * double(n));
+    JH_REQUIRE(g.su[0] == 0, "Gathered split tiles must hold source bit 0.");
+    const int64_t ntiles = int64_t(1) << g.nw;
+    const int vec = reinterpret_cast<uintptr_t>(src) % 16 == 0 ? 1 : 0;
+    hipLaunchKernelGGL(split_f16_gather_kernel, dim3((unsigned)std::min<int64_t>(ntiles, 65536)),
+                       dim3(256), 0, stream, static_cast<const float2 *>(src), g, dst, vec);
     JH_HIP(hipGetLastError());
 }
 

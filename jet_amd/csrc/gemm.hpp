@@ -53,6 +53,23 @@ inline int64_t splitk_partials(int64_t outputs, int64_t K)
 constexpr int kSplitRowPad = 16;
 constexpr int kSplitKPad = 32;
 
+// A split-FP16 operand read straight from an unpermuted source: the
+// permutation to [batch][rows][K] is a permutation of the linear index bits
+// (every extent a power of two), and one pass gathers LDS tiles made of all
+// K bits and `xr` row bits -- the row bits that hold the source's lowest
+// address bits, so reads are contiguous runs -- takes each tile row's
+// maximum and writes the four planes and the scale (no permuted copy).
+//   u: tile index in source order (bit j -> source bit su[j], ascending);
+//   v: tile index in target order (bits 0..log2k-1 = k, then the tile's row
+//      bits; u-bit j is v-bit u2v[j]; tile row bit i is row bit vrow[i]);
+//   w: tile number (bit j -> source bit wsrc[j] and row bit wrow[j]).
+constexpr int kGatherTileBits = 11; // 2048-element tiles (4096: 40% slower)
+struct SplitGather {
+    int tb = 0, log2k = 0, xr = 0, nw = 0;
+    int8_t su[16] = {}, u2v[16] = {}, vrow[16] = {};
+    int8_t wsrc[64] = {}, wrow[64] = {};
+};
+
 // Split-BF16 operand: planes re hi/lo, im hi/lo[, re+im hi/lo].
 //  * planar (3M kernel, six planes): plane p is [batch][rows_p][k_p] bf16 at
 //    offset p * plane_elems;
@@ -103,6 +120,9 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K
 void launch_split_f16(const void *src, int64_t rows, int64_t k, int64_t ld,
                       int64_t s, int64_t batch, const SplitOperand &dst,
                       hipStream_t stream);
+// The same planes and scales gathered from an unpermuted source (SplitGather).
+void launch_split_f16_gather(const void *src, const SplitGather &g, const SplitOperand &dst,
+                             hipStream_t stream);
 void launch_gemm_f16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                         void *C, int64_t M, int64_t N, int64_t ldc,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,

@@ -142,11 +142,11 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
     const int eb = d.elem_bytes;
     const void *At = a;
     const void *Bt = b;
-    if (d.need_a) {
+    if (d.need_a && !d.fuse_a) {
         launch_permute(d.pa, eb, a, ws + d.off_at, s);
         At = ws + d.off_at;
     }
-    if (d.need_b) {
+    if (d.need_b && !d.fuse_b) {
         launch_permute(d.pb, eb, b, ws + d.off_bt, s);
         Bt = ws + d.off_bt;
     }
@@ -165,8 +165,14 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
             sa.scale = reinterpret_cast<float *>(ws + d.off_scale_a);
             sb.rowmax = reinterpret_cast<uint32_t *>(ws + d.off_rmax_b);
             sb.scale = reinterpret_cast<float *>(ws + d.off_scale_b);
-            launch_split_f16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
-            launch_split_f16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
+            if (d.fuse_a)
+                launch_split_f16_gather(a, d.ga, sa, s);
+            else
+                launch_split_f16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
+            if (d.fuse_b)
+                launch_split_f16_gather(b, d.gb, sb, s);
+            else
+                launch_split_f16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
             launch_gemm_f16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
         }
         else {

@@ -193,3 +193,45 @@ extern "C" int network_permute_stats(int mode, int n_leaves, const int32_t *rank
     }
     return 0;
 }
+
+// Emulates split_f16_gather_kernel's index mapping on the CPU: for a source
+// tensor (row-major modes and extents) and the target order of its split
+// operand [rows][K], out[source index] = R * K + k, the target row and k that
+// the kernel writes the element to.  Returns 1 when the gather does not
+// apply, 0 otherwise (info = tb, xr, nw).
+extern "C" int split_gather_targets(int rank, const int32_t *modes, const int64_t *exts,
+                                    int trank, const int32_t *target, int64_t K, int64_t *out,
+                                    int *info)
+{
+    std::vector<int32_t> src(modes, modes + rank), tgt(target, target + trank);
+    std::vector<int64_t> ext(exts, exts + rank);
+    SplitGather g;
+    if (!plan_split_gather(src, ext, tgt, K, g)) return 1;
+    info[0] = g.tb;
+    info[1] = g.xr;
+    info[2] = g.nw;
+    const int64_t ntiles = int64_t(1) << g.nw, tsz = int64_t(1) << g.tb;
+    for (int64_t w = 0; w < ntiles; w++) {
+        int64_t sbase = 0, rbase = 0;
+        for (int j = 0; j < g.nw; j++)
+            if ((w >> j) & 1) {
+                sbase += int64_t(1) << g.wsrc[j];
+                rbase += int64_t(1) << g.wrow[j];
+            }
+        for (int64_t u = 0; u < tsz; u++) {
+            int64_t so = sbase;
+            int v = 0;
+            for (int j = 0; j < g.tb; j++)
+                if ((u >> j) & 1) {
+                    so += int64_t(1) << g.su[j];
+                    v |= 1 << g.u2v[j];
+                }
+            const int rl = v >> g.log2k, k = v & ((1 << g.log2k) - 1);
+            int64_t row = rbase;
+            for (int i = 0; i < g.xr; i++)
+                if ((rl >> i) & 1) row += int64_t(1) << g.vrow[i];
+            out[so] = row * K + k;
+        }
+    }
+    return 0;
+}

@@ -421,3 +421,36 @@ def test_split_f16_one_pass_groups(ctx, M, N, K):
     norm = np.abs(a).max(1)[:, None] * np.abs(b).max(1)[None, :] * np.sqrt(K)
     err = np.abs(got.reshape(M, N) - want) / norm
     assert err.max() < 1e-6, err.max()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_split_f16_gathered_operands(ctx, seed):
+    """Split-FP16 contractions of bond-2 tensors in scrambled mode orders:
+    the operand permutations are folded into the split pass (gathered LDS
+    tiles) wherever the plan allows it; results against numpy in FP64, and
+    against the same contraction with the operands permuted first (equal up
+    to the order of the K sum, which the plan takes from one operand)."""
+    rng = np.random.default_rng(100 + seed)
+    common = list(range(7 + seed % 2))
+    left = list(range(20, 28))
+    right = list(range(40, 47 + seed % 3))
+    ma = [int(x) for x in rng.permutation(common + left)]
+    mb = [int(x) for x in rng.permutation(common + right)]
+    a = rand_c(rng, 2 ** len(ma), np.complex64)
+    b = rand_c(rng, 2 ** len(mb), np.complex64)
+    mc = left + right
+    say(f"split-f16 gathered operands seed={seed} A{ma} B{mb}")
+    got, info = contract(ctx, a, ma, [2] * len(ma), b, mb, [2] * len(mb), mc,
+                         precision=nat.PREC_F16X3_4M)
+    assert info["kernel"] == "f16x3_4m_mfma", info
+    at = a.reshape([2] * len(ma)).transpose([ma.index(m) for m in left + common])
+    bt = b.reshape([2] * len(mb)).transpose([mb.index(m) for m in right + common])
+    A2 = at.reshape(2 ** len(left), -1).astype(np.complex128)
+    B2 = bt.reshape(2 ** len(right), -1).astype(np.complex128)
+    want = (A2 @ B2.T).reshape(-1)
+    assert rel_err(got, want) < 1e-6
+    # the same product from pre-permuted operands (no gather)
+    got2, _ = contract(ctx, np.ascontiguousarray(at).reshape(-1), left + common,
+                       [2] * len(ma), np.ascontiguousarray(bt).reshape(-1), right + common,
+                       [2] * len(mb), mc, precision=nat.PREC_F16X3_4M)
+    assert rel_err(got, got2) < 1e-6

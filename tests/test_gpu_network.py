@@ -99,3 +99,37 @@ def test_sliced_contraction(ctx, mode):
     amp = circuits.amplitude_statevector(circ)
     assert abs(total - amp) / abs(amp) < 1e-5
     assert abs(total - oracle.sum_compensated(got)) < 1e-6 * max(1.0, abs(total))
+
+
+def test_c2_plans_agree_in_fp64(ctx):
+    """The two committed plans of the 7x7 depth-14 amplitude (min-fill, and
+    refined for batched execution) are contraction orders of the same
+    sliced network: their first 16 slices agree in complex128 on FP64 MFMA
+    to 1e-10, and the split-FP16 run of the refined plan is within 1e-5 of
+    them (per-slice normwise).  Four sliced indices batched keep it small."""
+    import json
+    import os
+    root = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "jet_amd", "data")
+    circ = circuits.random_grid_circuit(7, 7, 14, seed=2026)
+    net = circuits.amplitude_network(circ)
+    res = {}
+    for name, prec, dtype in (("grid_7x7x14_seed2026_s12.json", "fp64", np.complex128),
+                              ("grid_7x7x14_seed2026_s12_refined.json", "fp64", np.complex128),
+                              ("grid_7x7x14_seed2026_s12_refined.json", "f16x3", np.complex64)):
+        with open(os.path.join(root, name)) as f:
+            plan = json.load(f)
+        sliced = plan["sliced"]
+        say(f"c2 plan {name} {prec}: 16 slices")
+        data = [d.astype(dtype) for d in net.data]
+        sc = SlicedContraction(net.indices, net.shapes, data, [tuple(p) for p in plan["path"]],
+                               sliced, batch=sliced[-4:], dtype=dtype, precision=prec, ctx=ctx,
+                               use_graph=False)
+        sc.run([0])
+        res[(name, prec)] = sc.slice_results(16).astype(np.complex128)
+        sc.net.close()
+    ref = res[("grid_7x7x14_seed2026_s12.json", "fp64")]
+    got64 = res[("grid_7x7x14_seed2026_s12_refined.json", "fp64")]
+    got16 = res[("grid_7x7x14_seed2026_s12_refined.json", "f16x3")]
+    assert np.linalg.norm(got64 - ref) / np.linalg.norm(ref) < 1e-10
+    assert np.linalg.norm(got16 - ref) / np.linalg.norm(ref) < 1e-5
