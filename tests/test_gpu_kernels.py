@@ -454,3 +454,29 @@ def test_split_f16_gathered_operands(ctx, seed):
                        [2] * len(ma), np.ascontiguousarray(bt).reshape(-1), right + common,
                        [2] * len(mb), mc, precision=nat.PREC_F16X3_4M)
     assert rel_err(got, got2) < 1e-6
+
+
+@pytest.mark.parametrize("dtype,prec", [(np.complex64, nat.PREC_FP32),
+                                        (np.complex64, nat.PREC_F16X3_4M),
+                                        (np.complex128, nat.PREC_FP64)])
+def test_split_k_scrambled_operands(ctx, dtype, prec):
+    """Few-output, long-K contractions (split-K GEMM, FP64 partial sums) of
+    bond-2 tensors in scrambled mode orders, with a batch mode (the last
+    step of a batched amplitude network); results against numpy in FP64."""
+    rng = np.random.default_rng(7)
+    common = list(range(16))
+    ma = [int(x) for x in rng.permutation(common + [30, 40])]    # 30: left, 40: batch
+    mb = [int(x) for x in rng.permutation(common + [40])]
+    a = rand_c(rng, 2 ** len(ma), dtype)
+    b = rand_c(rng, 2 ** len(mb), dtype)
+    mc = [40, 30]
+    say(f"split-k scrambled operands {np.dtype(dtype).name} prec={prec}")
+    got, info = contract(ctx, a, ma, [2] * len(ma), b, mb, [2] * len(mb), mc, precision=prec)
+    assert info["kernel"] == "splitk", info
+    at = a.reshape([2] * len(ma)).transpose([ma.index(m) for m in mc + common])
+    bt = b.reshape([2] * len(mb)).transpose([mb.index(m) for m in [40] + common])
+    A3 = at.reshape(2, 2, -1).astype(np.complex128)
+    B2 = bt.reshape(2, -1).astype(np.complex128)
+    want = np.einsum("bmk,bk->bm", A3, B2).reshape(-1)
+    tol = 1e-12 if dtype == np.complex128 else 1e-6
+    assert rel_err(got, want) < tol
