@@ -5,6 +5,7 @@ the tolerances stated per test (normwise relative error against the FP64
 oracle): 1e-5 for complex64 on FP32 MFMA and on split-BF16, 1e-12 for
 complex128 on FP64 MFMA.
 """
+import ctypes
 import string
 
 import numpy as np
@@ -14,7 +15,7 @@ import oracle
 from jet_amd import _native as nat
 from jet_amd import Permuter, Tensor
 
-from hiputil import contract, permute, rand_c, rel_err, say
+from hiputil import contract, from_device, permute, rand_c, rel_err, say
 
 pytestmark = pytest.mark.gpu
 
@@ -480,3 +481,41 @@ def test_split_k_scrambled_operands(ctx, dtype, prec):
     want = np.einsum("bmk,bk->bm", A3, B2).reshape(-1)
     tol = 1e-12 if dtype == np.complex128 else 1e-6
     assert rel_err(got, want) < tol
+
+
+def test_split_f16_unaligned_operands(ctx):
+    """Operands at 8-byte (not 16-byte) aligned addresses, as a caller's
+    complex64 sub-array may be: the split passes fall back from 16-byte
+    loads, in the gathered split (scrambled B) and in the one-pass split
+    (A in order); results equal those from aligned copies."""
+    rng = np.random.default_rng(77)
+    common = list(range(8))
+    left = list(range(20, 28))
+    right = list(range(40, 48))
+    ma = left + common
+    mb = [int(x) for x in rng.permutation(common + right)]
+    a = rand_c(rng, 2 ** len(ma), np.complex64)
+    b = rand_c(rng, 2 ** len(mb), np.complex64)
+    mc = left + right
+    say("split-f16 unaligned operands")
+    want, info = contract(ctx, a, ma, [2] * len(ma), b, mb, [2] * len(mb), mc,
+                          precision=nat.PREC_F16X3_4M)
+    assert info["kernel"] == "f16x3_4m_mfma", info
+    plan = ctypes.c_void_p()
+    nat.check(nat.lib().jet_hip_contraction_plan_create(
+        ctx.handle, nat.C64, nat.PREC_F16X3_4M, nat.ROW_MAJOR, len(ma), nat.i32(ma),
+        nat.i64([2] * len(ma)), len(mb), nat.i32(mb), nat.i64([2] * len(mb)), len(mc),
+        nat.i32(mc), ctypes.byref(plan)))
+    try:
+        da = nat.DeviceBuffer(a.nbytes + 16, ctx)
+        db = nat.DeviceBuffer(b.nbytes + 16, ctx)
+        nat.check(nat.lib().jet_hip_memcpy(ctx.handle, da.ptr + 8, a.ctypes.data, a.nbytes, 0,
+                                           None, 0))
+        nat.check(nat.lib().jet_hip_memcpy(ctx.handle, db.ptr + 8, b.ctypes.data, b.nbytes, 0,
+                                           None, 0))
+        dc = nat.DeviceBuffer(8 * 2 ** len(mc), ctx)
+        nat.check(nat.lib().jet_hip_contract(plan, da.ptr + 8, db.ptr + 8, dc.ptr, None, 0, None))
+        got = from_device(dc, 2 ** len(mc), np.complex64)
+    finally:
+        nat.lib().jet_hip_plan_destroy(plan)
+    assert np.array_equal(got, want)
