@@ -158,3 +158,23 @@ def test_committed_c2_plans_are_consistent():
     assert [t0.total(), t1.total()] == pytest.approx(info["modelled_execution_s"], rel=1e-12)
     assert t1.total() < 0.5 * t0.total()
     assert t1.max_log2size() <= t0.max_log2size() + info["extra_log2"]
+
+
+def test_refine_batched_path_small_circuit():
+    """refine_batched_path on a 5x5 depth-10 amplitude sliced on 6 indices,
+    3 of them batched: a valid SSA path of the same network whose modelled
+    batched execution is no slower than the input's, within the allowed
+    growth of the largest batched intermediate."""
+    circ = circuits.random_grid_circuit(5, 5, 10, seed=4)
+    net = circuits.amplitude_network(circ)
+    dims = {l: 2 for ix in net.indices for l in ix}
+    path = paths.best_greedy_path(net.indices, dims, trials=1)
+    sliced = paths.choose_slices_for_flops(net.indices, dims, path, 6, 14)
+    t0 = paths.batched_tree(net.indices, dims, path, sliced, 3)
+    p2 = paths.refine_batched_path(net.indices, dims, path, sliced, 3, extra_log2=1.0,
+                                   subtree_size=6, rounds=2)
+    _valid_ssa(len(net.indices), p2)
+    t1 = paths.batched_tree(net.indices, dims, p2, sliced, 3)
+    assert t1.total() <= t0.total() * (1 + 1e-12)
+    assert t1.max_log2size() <= t0.max_log2size() + 1.0
+    assert t1.mask[t1.root] == t1.open        # only the batched indices remain open
