@@ -365,6 +365,20 @@ def c2_problem(plan_name="min-fill"):
     return plan, circ, net
 
 
+C2_AMPLITUDE = os.path.join(ROOT, "tests", "golden", "c2_amplitude_fp64.json")
+
+
+def c2_reference_amplitude(plan_name):
+    """The complex128 FP64-MFMA amplitude of all 4096 slices (any plan: the
+    sliced network is the same), as tests/golden/make_c2_amplitude.py wrote
+    it; None if that file is absent."""
+    if not os.path.exists(C2_AMPLITUDE):
+        return None
+    with open(C2_AMPLITUDE) as f:
+        doc = json.load(f)
+    return complex(*doc["amplitude"])
+
+
 def run_c2(args):
     from jet_amd import _native as nat
     from jet_amd.network import SlicedContraction
@@ -386,15 +400,20 @@ def run_c2(args):
 
     def timed(sc, iters, warmup, profile):
         """The timed executions run as one captured hipGraph launch (the
-        warm-up executions as another, launched first).  With `profile`, one
-        more execution runs eagerly under the event profiler afterwards: the
-        per-kernel times of the roofline."""
+        warm-up executions as another, launched first), followed on the same
+        stream by the FP64 compensated sum of the per-slice results, inside
+        the timed region.  Execution i covers slice range
+        execution_slot(i, n_loop): past the last one the slice loop starts
+        again.  With `profile`, one more execution runs eagerly under the
+        event profiler afterwards: the per-kernel times of the roofline."""
+        done = min(sc.n_loop, warmup + iters) * sc.n_batch
         if args.eager:
             sc.run(range(warmup))
             ctx.synchronize()
             e0, e1 = nat.Event(ctx), nat.Event(ctx)
             e0.record()
             sc.run(range(warmup, warmup + iters))
+            sc.reduce_enqueue(done)
             e1.record()
             ctx.synchronize()
             ms = e0.elapsed_ms(e1)
@@ -409,10 +428,12 @@ def run_c2(args):
             e0, e1 = nat.Event(ctx), nat.Event(ctx)
             e0.record()
             tb.launch()
+            sc.reduce_enqueue(done)
             e1.record()
             ctx.synchronize()
             ms = e0.elapsed_ms(e1)
             tb.close()
+        total = sc.reduced()
         prof = None
         if profile:
             nat.Profiler.reset()
@@ -421,19 +442,25 @@ def run_c2(args):
             ctx.synchronize()
             prof = nat.Profiler.read()
             nat.Profiler.enable(False)
-        return ms, prof
+        return ms, prof, total, done
 
     steps, warm = args.steps, args.warmup
     label = SPLIT_LABEL[args.split]
     sc16 = build(SPLIT_MODES[args.split][1], np.complex64)
-    n_batch = sc16.n_batch
+    n_batch, n_slices = sc16.n_batch, sc16.n_slices
     exec_flops = sc16.flops_per_execution
     log(f"c2: {label} network: {exec_flops:.3e} flops/execution, arena "
-        f"{sc16.net.arena_bytes / 2**30:.1f} GiB, workspace {sc16.net.workspace_bytes / 2**30:.1f} GiB")
-    ms16, prof16 = timed(sc16, steps, warm, True)
-    res16 = sc16.slice_results((warm + steps) * n_batch)
+        f"{sc16.net.arena_bytes / 2**30:.1f} GiB, workspace {sc16.net.workspace_bytes / 2**30:.1f} GiB, "
+        f"{sc16.n_loop} executions of {n_batch} slices per amplitude")
+    ms16, prof16, sum16, done16 = timed(sc16, steps, warm, True)
+    if done16 < n_slices:
+        # The whole amplitude needs every execution: run the rest untimed.
+        sc16.run(range(warm + steps, sc16.n_loop))
+        sum16 = sc16.reduce()
+    res16 = sc16.slice_results()
     log(f"c2: {label} {ms16 / steps:.1f} ms/execution, "
         f"{exec_flops * steps / ms16 / 1e9:.1f} TFLOP/s")
+    sc16.net.close()
     del sc16
 
     # The other split modes, timed the same way for comparison.
@@ -442,18 +469,20 @@ def run_c2(args):
         if mode == args.split:
             continue
         sc = build(SPLIT_MODES[mode][1], np.complex64)
-        ms, _ = timed(sc, steps, warm, False)
+        ms, _, _, _ = timed(sc, steps, warm, False)
         others[mode] = (ms, sc.slice_results(n_batch))
         log(f"c2: {SPLIT_LABEL[mode]} {ms / steps:.1f} ms/execution, "
             f"{exec_flops * steps / ms / 1e9:.1f} TFLOP/s")
+        sc.net.close()
         del sc
 
     steps32 = max(1, args.steps32)
     sc32 = build("fp32", np.complex64)
-    ms32, prof32 = timed(sc32, steps32, 1, True)
-    res32 = sc32.slice_results(steps32 * n_batch)
+    ms32, prof32, _, _ = timed(sc32, steps32, 1, True)
+    res32 = sc32.slice_results(n_batch)
     log(f"c2: fp32 {ms32 / steps32:.1f} ms/execution, "
         f"{exec_flops * steps32 / ms32 / 1e9:.1f} TFLOP/s")
+    sc32.net.close()
     del sc32
 
     # FP64 reference (complex128, FP64 MFMA) of the first execution's first
@@ -463,18 +492,20 @@ def run_c2(args):
     batch64 = batch[2:] if len(batch) >= 2 else batch
     sc64 = build("fp64", np.complex128, batch64)
     n_ref = sc64.n_batch
-    ms64, _ = timed(sc64, 1, 0, False)
+    ms64, _, _, _ = timed(sc64, 1, 0, False)
     res64 = sc64.slice_results(n_ref)
-    sum64 = sc64.reduce(n_ref)
+    sc64.net.close()
     del sc64
     err16 = float(np.linalg.norm(res16[:n_ref] - res64) / np.linalg.norm(res64))
     err32 = float(np.linalg.norm(res32[:n_ref] - res64) / np.linalg.norm(res64))
-    amp16 = complex(np.sum(res16[:n_ref].astype(np.complex128)))
-    amp_err16 = abs(amp16 - sum64) / abs(sum64)
-    amp32 = complex(np.sum(res32[:n_ref].astype(np.complex128)))
-    amp_err32 = abs(amp32 - sum64) / abs(sum64)
+    worst16 = float(np.max(np.abs(res16[:n_ref] - res64)) / np.max(np.abs(res64)))
+    # The whole 4096-slice amplitude against the complex128 FP64-MFMA
+    # amplitude of the same plan (recomputed live by --full-amplitude).
+    amp64 = c2_reference_amplitude(args.c2_plan)
+    amp_err16 = abs(sum16 - amp64) / abs(amp64) if amp64 is not None else None
     log(f"c2: fp64 {ms64:.1f} ms/execution; rel err per-slice {label} {err16:.2e} "
-        f"fp32 {err32:.2e}; partial-amplitude err {amp_err16:.2e}")
+        f"fp32 {err32:.2e}; whole-amplitude {label} "
+        f"{amp_err16 if amp_err16 is None else f'{amp_err16:.2e}'}")
     compare = {}
     for mode, (ms, res) in others.items():
         e = float(np.linalg.norm(res[:n_ref] - res64) / np.linalg.norm(res64))
@@ -529,9 +560,12 @@ def run_c2(args):
         "speedup_split_vs_fp32": round(tflops16 / tflops32, 3),
         "projected_amplitude_s": {"split": round(4096 / sps16, 1), "fp32": round(4096 / sps32, 1)},
         "rel_err_vs_fp64": {"split_per_slice": err16, "fp32_per_slice": err32,
-                            "split_partial_amplitude": amp_err16,
-                            "fp32_partial_amplitude": amp_err32,
-                            "slices_compared": n_ref},
+                            "split_worst_slice_vs_max": worst16,
+                            "slices_compared": n_ref,
+                            "split_whole_amplitude": amp_err16,
+                            "amplitude_split": [sum16.real, sum16.imag],
+                            "amplitude_fp64": (None if amp64 is None
+                                               else [amp64.real, amp64.imag])},
         "other_split_modes": compare,
         "other_plan": (c2_plan_leg(args, ctx, net, label, other) if args.other_plan
                        else None),
@@ -943,7 +977,8 @@ def main():
         args.no_cpu_baseline = True
     else:
         result, host = run_c2(args)
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    # Every host core this process may run on.
+    threads = args.cpu_threads or len(os.sched_getaffinity(0))
     if not args.no_cpu_baseline:
         log("cpu baseline")
         result["cpu_baseline"] = (c1_cpu_baseline(host, threads) if args.workload == "c1"

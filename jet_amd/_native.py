@@ -198,14 +198,17 @@ class Context:
 _default_ctx: Optional[Context] = None
 
 
+_ctx_lock = threading.Lock()
+
+
 def default_context() -> Context:
+    """The process-wide context of device JET_HIP_DEVICE (default 0), created
+    once under a lock so concurrent first callers share it."""
     global _default_ctx
-    with _lock:
+    with _ctx_lock:
         if _default_ctx is None:
-            pass
-    if _default_ctx is None:
-        _default_ctx = Context(int(os.environ.get("JET_HIP_DEVICE", "0")))
-    return _default_ctx
+            _default_ctx = Context(int(os.environ.get("JET_HIP_DEVICE", "0")))
+        return _default_ctx
 
 
 class DeviceBuffer:
@@ -221,12 +224,18 @@ class DeviceBuffer:
     def upload(self, host, stream=None) -> None:
         import numpy as np
         host = np.ascontiguousarray(host)
-        assert host.nbytes <= self.nbytes
+        if self.ptr is None or host.nbytes > self.nbytes:
+            raise JetHipError(f"Error in Jet: upload of {host.nbytes} bytes into a "
+                              f"{self.nbytes}-byte device buffer.")
         check(lib().jet_hip_memcpy(self.ctx.handle, self.ptr, host.ctypes.data, host.nbytes, 0,
                                    stream, 0))
 
     def download(self, host, stream=None) -> None:
-        assert host.flags["C_CONTIGUOUS"] and host.nbytes <= self.nbytes
+        if not host.flags["C_CONTIGUOUS"]:
+            raise JetHipError("Error in Jet: download needs a C-contiguous host array.")
+        if self.ptr is None or host.nbytes > self.nbytes:
+            raise JetHipError(f"Error in Jet: download of {host.nbytes} bytes from a "
+                              f"{self.nbytes}-byte device buffer.")
         check(lib().jet_hip_memcpy(self.ctx.handle, host.ctypes.data, self.ptr, host.nbytes, 1,
                                    stream, 0))
 

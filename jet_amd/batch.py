@@ -234,8 +234,7 @@ class UnevenBatch:
             else:
                 sc = part
                 for i in range(job.first_execution, job.first_execution + job.executions):
-                    out = sc.results.ptr + i * sc.n_batch * self.dtype.itemsize
-                    ex.append((sc.net, sc._leaf_ptrs(sc.loop_values(i)), out))
+                    ex.append((sc.net, sc._leaf_ptrs(sc.loop_values(i)), sc.result_ptr(i)))
         return ex
 
     def launch(self) -> None:

@@ -226,6 +226,24 @@ void run_network(jet_hip_network_s *net, const void *const *leaves, void *out, h
     }
 }
 
+// hipMalloc that first returns the stream-ordered pool's cached blocks to
+// the device when the request does not fit.
+void *device_alloc(size_t bytes, int device)
+{
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        JH_HIP(hipDeviceSynchronize());
+        hipMemPool_t pool;
+        JH_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+        JH_HIP(hipMemPoolTrimTo(pool, 0));
+        e = hipMalloc(&p, bytes);
+    }
+    JH_HIP(e);
+    return p;
+}
+
 void free_graph(jet_hip_network_s *net)
 {
     if (net->graph_exec) (void)hipGraphExecDestroy(net->graph_exec);
@@ -308,8 +326,19 @@ int jet_hip_malloc(jet_hip_ctx_t ctx, size_t bytes, void **ptr)
         *ptr = nullptr;
         if (bytes == 0) bytes = 1;
         // Stream-ordered pool allocation on the context stream, made visible
-        // to every stream before returning.
-        JH_HIP(hipMallocAsync(ptr, bytes, ctx->stream));
+        // to every stream before returning.  The pool keeps freed blocks
+        // cached; when a request does not fit, the cache is released to the
+        // device and the request made once more.
+        hipError_t e = hipMallocAsync(ptr, bytes, ctx->stream);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            JH_HIP(hipDeviceSynchronize());
+            hipMemPool_t pool;
+            JH_HIP(hipDeviceGetDefaultMemPool(&pool, ctx->device));
+            JH_HIP(hipMemPoolTrimTo(pool, 0));
+            e = hipMallocAsync(ptr, bytes, ctx->stream);
+        }
+        JH_HIP(e);
         JH_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
@@ -630,16 +659,11 @@ int jet_hip_network_create(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, jet_hip_pre
             n->plan = plan_network(elem_bytes_of(dtype), mode_of(dtype, precision), n_leaves,
                                    leaf_ranks, leaf_modes, leaf_extents, n_steps, path, n_out,
                                    out_modes);
-            if (n->plan.arena_bytes) {
-                void *p = nullptr;
-                JH_HIP(hipMalloc(&p, n->plan.arena_bytes));
-                n->arena = static_cast<char *>(p);
-            }
-            if (n->plan.workspace_bytes) {
-                void *p = nullptr;
-                JH_HIP(hipMalloc(&p, n->plan.workspace_bytes));
-                n->workspace = static_cast<char *>(p);
-            }
+            if (n->plan.arena_bytes)
+                n->arena = static_cast<char *>(device_alloc(n->plan.arena_bytes, ctx->device));
+            if (n->plan.workspace_bytes)
+                n->workspace =
+                    static_cast<char *>(device_alloc(n->plan.workspace_bytes, ctx->device));
         }
         catch (...) {
             if (n->arena) (void)hipFree(n->arena);
@@ -681,16 +705,11 @@ int jet_hip_dag_create(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype, jet_hip_precisi
             n->plan = plan_dag(elem_bytes_of(dtype), mode_of(dtype, precision), n_leaves,
                                leaf_ranks, leaf_modes, leaf_extents, n_steps, steps, n_keep,
                                keep);
-            if (n->plan.arena_bytes) {
-                void *p = nullptr;
-                JH_HIP(hipMalloc(&p, n->plan.arena_bytes));
-                n->arena = static_cast<char *>(p);
-            }
-            if (n->plan.workspace_bytes) {
-                void *p = nullptr;
-                JH_HIP(hipMalloc(&p, n->plan.workspace_bytes));
-                n->workspace = static_cast<char *>(p);
-            }
+            if (n->plan.arena_bytes)
+                n->arena = static_cast<char *>(device_alloc(n->plan.arena_bytes, ctx->device));
+            if (n->plan.workspace_bytes)
+                n->workspace =
+                    static_cast<char *>(device_alloc(n->plan.workspace_bytes, ctx->device));
         }
         catch (...) {
             if (n->arena) (void)hipFree(n->arena);
@@ -788,6 +807,9 @@ int jet_hip_network_capture(jet_hip_network_t net, const void *const *leaves, vo
         JH_REQUIRE(net && leaves && (out || net->plan.dag), "Null argument.");
         set_device(net->ctx);
         hipStream_t s = pick_stream(net->ctx, stream);
+        // A previous graph may still run (launched on any stream): wait for
+        // it before its exec is destroyed, as jet_hip_network_destroy does.
+        if (net->graph_exec) JH_HIP(hipDeviceSynchronize());
         free_graph(net);
         const bool prof_on = prof::enabled();
         prof::set_enabled(false);

@@ -33,6 +33,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _native as nat
+from .abort import JetError
 
 _PREC = {None: nat.PREC_DEFAULT, "default": nat.PREC_DEFAULT, "fp32": nat.PREC_FP32,
          "bf16x3": nat.PREC_BF16X3, "fp64": nat.PREC_FP64, "bf16x3_4m": nat.PREC_BF16X3_4M,
@@ -249,6 +250,19 @@ def dag_estimate(indices, shapes, steps, keep, dtype=np.complex64,
     return fl.value, ar.value, ws.value
 
 
+def execution_slot(i: int, n_loop: int) -> int:
+    """Loop-value combination (and result slot) of loop iteration i of a
+    sliced contraction with n_loop combinations: iterations past the last
+    combination start the slice loop again, so slot and leaves always agree
+    and every write stays inside the n_loop * n_batch results."""
+    i, n_loop = int(i), int(n_loop)
+    if n_loop < 1:
+        raise JetError("A sliced contraction has at least one loop iteration.")
+    if i < 0:
+        raise JetError(f"Loop iteration {i} is negative.")
+    return i % n_loop
+
+
 def _slice_leaf(labels, shape, data, fixed: Dict[str, int]):
     arr = np.asarray(data).reshape(shape) if shape else np.asarray(data).reshape(())
     keep_l, keep_s, idx = [], [], []
@@ -321,18 +335,27 @@ class SlicedContraction:
                 for var, held in zip(self._variants, self._leaf_loop)]
 
     def loop_values(self, i: int) -> Dict[str, int]:
-        vals, rem = {}, i
+        """Values of the loop indices at loop iteration i (last loop index
+        fastest, as UnravelIndex orders them)."""
+        vals, rem = {}, execution_slot(i, self.n_loop)
         for l in reversed(self.loop):
             vals[l] = rem % self.dims[l]
             rem //= self.dims[l]
         return vals
 
+    def result_ptr(self, i: int) -> int:
+        """Device address of the n_batch per-slice results of loop iteration i."""
+        return self.results.ptr + execution_slot(i, self.n_loop) * self.n_batch * \
+            self.dtype.itemsize
+
     def run(self, loop_iterations: Optional[Sequence[int]] = None) -> None:
-        """Execute the given loop iterations (all by default); each writes its
-        n_batch per-slice results into the results vector."""
+        """Execute the given loop iterations (all by default); iteration i
+        runs loop-value combination ``execution_slot(i, n_loop)`` and writes
+        its n_batch per-slice results into that slot of the results vector,
+        so iterations past the last combination repeat the slice loop."""
         its = range(self.n_loop) if loop_iterations is None else loop_iterations
         for i in its:
-            out = self.results.ptr + i * self.n_batch * self.dtype.itemsize
+            out = self.result_ptr(i)
             ptrs = self._leaf_ptrs(self.loop_values(i))
             if self.use_graph:
                 self.net.launch(ptrs, out)
@@ -342,27 +365,42 @@ class SlicedContraction:
     def capture(self, loop_iterations: Sequence[int]):
         """One hipGraph over the given loop iterations (jet_amd.batch.Batch):
         a single launch then runs all of them, each writing its n_batch
-        per-slice results."""
+        per-slice results (slots as in ``run``)."""
         from .batch import Batch
-        ex = [(self.net, self._leaf_ptrs(self.loop_values(i)),
-               self.results.ptr + i * self.n_batch * self.dtype.itemsize)
+        ex = [(self.net, self._leaf_ptrs(self.loop_values(i)), self.result_ptr(i))
               for i in loop_iterations]
         return Batch(ex, self.ctx)
 
+    def _count(self, count: Optional[int]) -> int:
+        n = self.n_slices if count is None else int(count)
+        if not 0 <= n <= self.n_slices:
+            raise JetError(f"Slice count {n} is outside [0, {self.n_slices}].")
+        return n
+
     def slice_results(self, count: Optional[int] = None) -> np.ndarray:
-        n = self.n_slices if count is None else count
+        """The first `count` per-slice results (all by default), in slice order."""
+        n = self._count(count)
         host = np.empty(n, dtype=self.dtype)
         self.results.download(host)
         return host
 
-    def reduce(self, count: Optional[int] = None) -> complex:
-        """FP64 compensated sum of the first `count` per-slice results."""
-        n = self.n_slices if count is None else count
+    def reduce_enqueue(self, count: Optional[int] = None, stream=None) -> None:
+        """Queue the FP64 compensated sum of the first `count` per-slice
+        results on the stream (no wait); ``reduced()`` reads it back."""
+        n = self._count(count)
         nat.check(nat.lib().jet_hip_reduce_compensated(self.ctx.handle, self.code, n,
-                                                       self.results.ptr, self._sum.ptr, None))
+                                                       self.results.ptr, self._sum.ptr, stream))
+
+    def reduced(self) -> complex:
+        """The last queued compensated sum (waits for it)."""
         host = np.empty(2, dtype=np.float64)
         self._sum.download(host)
         return complex(host[0], host[1])
+
+    def reduce(self, count: Optional[int] = None) -> complex:
+        """FP64 compensated sum of the first `count` per-slice results."""
+        self.reduce_enqueue(count)
+        return self.reduced()
 
 
 def choose_batch(indices, shapes, path, sliced: Sequence[str], budget_bytes: float,

@@ -82,3 +82,34 @@ int emu_permute(int elem, int rank, const int64_t *extents, const int32_t *perm,
 }
 
 } // extern "C"
+
+extern "C" {
+
+// The bit plan of a permutation without data: nbits, tb, ti, nr and the
+// tile/outer bit maps (arrays of 16, 16, 16, 64, 64 entries).  Returns the
+// plan kind (0 copy, 1 bits, 2 generic) or -1 for a planner error.
+int emu_bits_plan(int rank, const int64_t *extents, const int32_t *perm, int *scalars,
+                  int8_t *tile_in, int8_t *tile_out, int8_t *v2u, int8_t *outer_in,
+                  int8_t *outer_out)
+{
+    PermutePlan plan;
+    try {
+        plan = make_permute_plan(rank, extents, perm);
+    }
+    catch (const Error &) {
+        return -1;
+    }
+    const BitPermParams &p = plan.bits;
+    scalars[0] = p.nbits;
+    scalars[1] = p.tb;
+    scalars[2] = p.ti;
+    scalars[3] = p.nr;
+    std::memcpy(tile_in, p.tile_in, 16);
+    std::memcpy(tile_out, p.tile_out, 16);
+    std::memcpy(v2u, p.v2u, 16);
+    std::memcpy(outer_in, p.outer_in, 64);
+    std::memcpy(outer_out, p.outer_out, 64);
+    return (int)plan.kind;
+}
+
+} // extern "C"

@@ -100,3 +100,40 @@ def test_permute_plan_rank28_bond2():
     rc, out, kind, tb = _emulate(L, x, shape, perm)
     assert rc == 0 and kind == 1 and tb == 11
     assert np.array_equal(out, oracle.permute_default(x, shape, perm))
+
+
+def _bits_plan(L, shape, perm):
+    sc = (ctypes.c_int * 4)()
+    arrs = [(ctypes.c_int8 * n)() for n in (16, 16, 16, 64, 64)]
+    kind = L.emu_bits_plan(len(shape), _native.i64(shape), _native.i32(perm), sc, *arrs)
+    nbits, tb, ti, nr = list(sc)
+    tile_in, tile_out, v2u, outer_in, outer_out = [list(a) for a in arrs]
+    return kind, nbits, tb, ti, nr, tile_in[:tb], tile_out[:tb], v2u[:tb], outer_in[:nr], \
+        outer_out[:nr]
+
+
+@pytest.mark.parametrize("rank", [24, 28, 33, 36, 40, 48, 62])
+def test_bit_plans_of_large_bond2_permutations(rank):
+    """For bond-2 tensors of rank up to 62 (a rank-40 complex64 tensor is
+    8 TiB, beyond one GPU; rank 33, 64 GiB, runs in the GPU tests), the bit
+    plan the kernels execute is exactly the permutation: input bit
+    rank-1-perm[i] goes to output bit rank-1-i.  Tile and outer bits
+    partition the input bits, outer bits and tile bits (through v2u) map by
+    that permutation, the tile's low input bits are contiguous and its size
+    fits LDS -- so every load and store is in bounds and every output
+    element is written once, as the kernels' index helpers compute them."""
+    L = _emu_lib()
+    rng = np.random.default_rng(rank)
+    for trial in range(20):
+        perm = [int(p) for p in rng.permutation(rank)]
+        if trial == 0:
+            perm = list(reversed(range(rank)))
+        kind, nbits, tb, ti, nr, t_in, t_out, v2u, o_in, o_out = _bits_plan(L, [2] * rank, perm)
+        assert kind == 1 and nbits == rank and tb == 11 and tb + nr == nbits
+        sigma = {rank - 1 - perm[i]: rank - 1 - i for i in range(rank)}
+        assert sorted(t_in + o_in) == list(range(nbits))
+        assert t_in == sorted(t_in) and t_in[:ti] == list(range(ti)) and ti >= 1
+        assert o_out == [sigma[b] for b in o_in]
+        assert sorted(v2u) == list(range(tb))
+        assert t_out == [sigma[t_in[u]] for u in v2u]
+        assert t_out == sorted(t_out)

@@ -13,6 +13,7 @@ import re
 import numpy as np
 import pytest
 
+import oracle
 from hosttensor import HostTensor
 from jet_amd import PathInfo, TensorNetwork
 from jet_amd.tensor_network_io import (TensorFileException, TensorNetworkFile,
@@ -127,3 +128,19 @@ def test_reference_network_files(name, n_tensors, n_steps, pathinfo_flops, max_r
     for a, b in zip(again["tensors"], orig["tensors"]):
         assert a[:3] == b[:3]
         np.testing.assert_allclose(np.array(a[3]), np.array(b[3]), rtol=1e-7, atol=1e-7)
+
+
+def test_m10_fixture_slices_sum_to_whole():
+    """The committed m10 oracle values are consistent: summing the 16
+    slices over p7, s7, h4, m1 restores the unsliced contraction
+    (TensorNetwork::SliceIndices keeps every term of the sum once)."""
+    import json
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    with open(os.path.join(golden, "m10_values.json")) as f:
+        vals = json.load(f)
+    total = oracle.sum_compensated(np.array([complex(*v) for v in vals["slice_values"]]))
+    whole = complex(*vals["whole"])
+    assert abs(total - whole) / abs(whole) < 1e-12
+    with open(os.path.join(golden, "m10.json")) as f:
+        doc = json.load(f)
+    assert len(doc["tensors"]) == 322 and len(doc["path"]) == 321
