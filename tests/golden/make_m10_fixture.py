@@ -5,7 +5,9 @@ Copies examples/paper_benchmarks/data_files/m10.json of the Jet source tree
 m=10 circuit and a 321-step SSA contraction path, read as text) to
 tests/golden/m10.json, and writes tests/golden/m10_values.json:
 
-  * the whole network contracted along the stored path (a scalar), and
+  * the whole network contracted along the stored path (a scalar), from the
+    complex64-rounded data and from the file's values read in complex128,
+    and
   * the 16 slices over the first four entries of the benchmark's sliced-
     index list (examples/paper_benchmarks/CPU/jet_cpu_m10/jet_sliced.cpp:
     53-54: p7, s7, h4, m1), slice s fixing them to UnravelIndex(s, [2]*4),
@@ -59,12 +61,19 @@ def main():
     labels, _, full = oracle.contract_path(tensors, path)
     assert labels == [], labels
     print(f"whole network: {complex(full[0])!r}", flush=True)
+    # The file's decimal values read straight into complex128 (what a
+    # complex128 TensorNetworkSerializer holds), not rounded to complex64.
+    exact = [(list(t[1]), list(t[2]), np.array([complex(re, im) for re, im in t[3]]))
+             for t in doc["tensors"]]
+    _, _, full128 = oracle.contract_path(exact, path)
+    print(f"whole network from unrounded data: {complex(full128[0])!r}", flush=True)
     vals = slice_values(net, path, SLICED, list(range(16)), log=lambda m: print(m, flush=True))
     total = oracle.sum_compensated(np.array(vals))
     print(f"sum of slices {total!r}", flush=True)
     with open(OUT, "w") as f:
         json.dump({"network": "tests/golden/m10.json",
                    "whole": [full[0].real, full[0].imag],
+                   "whole_unrounded": [full128[0].real, full128[0].imag],
                    "sliced": SLICED,
                    "slice_values": [[v.real, v.imag] for v in vals],
                    "made_by": "tests/golden/make_m10_fixture.py (oracle.contract in complex128)"},

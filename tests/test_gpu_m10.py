@@ -57,7 +57,8 @@ def test_m10_serializer_and_contract_path(ctx, m10, dtype, tol):
     assert f.tensors.num_tensors == 322 and len(f.path.path) == 321
     result = f.tensors.contract(f.path.path)
     assert result.indices == []
-    want = complex(*vals["whole"])
+    # A complex128 serializer keeps the file's decimal values unrounded.
+    want = complex(*vals["whole" if dtype == np.complex64 else "whole_unrounded"])
     assert _rel(result.get_data()[0], want) < tol
 
 
