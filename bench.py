@@ -867,7 +867,8 @@ def run_c4(args):
     """Gradient of L = |<b|C(theta)|0>|^2 for a seeded 6x6 depth-10 grid
     circuit whose ~300 fSim angles are ~U(0, 2 pi): one forward contraction
     plus the adjoint (environment) pass on the GPU, planned as one
-    contraction DAG and replayed as a hipGraph, through the
+    contraction DAG (every --c4-checkpoint-th forward intermediate kept, the
+    others recomputed inside it) and replayed as a hipGraph, through the
     torch.autograd.Function; a step is one loss + gradient evaluation.
     Checked against central differences (h = 1e-3, complex128 on FP64 MFMA)
     on --c4-fd angles (all of them by default)."""
@@ -879,21 +880,42 @@ def run_c4(args):
     net = circuits.amplitude_network(circ)
     dims = {l: 2 for ix in net.indices for l in ix}
     path = paths.best_minfill_path(net.indices, dims, trials=8, seed=1)
-    prob = ParameterizedAmplitude(circ, path)
-    theta = torch.tensor(prob.theta, dtype=torch.float64, requires_grad=True)
-    ctx = prob.ctx
-    for _ in range(args.warmup):
-        theta.grad = None
-        AmplitudeLoss.apply(theta, prob).backward()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        theta.grad = None
-        loss = AmplitudeLoss.apply(theta, prob)
-        loss.backward()
-    ctx.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    g = theta.grad.numpy().copy()
+
+    def timed(prob):
+        theta = torch.tensor(prob.theta, dtype=torch.float64, requires_grad=True)
+        for _ in range(args.warmup):
+            theta.grad = None
+            AmplitudeLoss.apply(theta, prob).backward()
+        prob.ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            theta.grad = None
+            loss = AmplitudeLoss.apply(theta, prob)
+            loss.backward()
+        prob.ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, theta.grad.numpy().copy()
+
+    # Checkpointed recomputation inside the DAG against keeping every
+    # forward intermediate: arena bytes and time per loss + gradient.
+    checkpointing = {}
+    for k in sorted({0, 2, 4, args.c4_checkpoint}):
+        if k == args.c4_checkpoint:
+            continue
+        p = ParameterizedAmplitude(circ, path, checkpoint_every=k)
+        ms_k, _ = timed(p)
+        checkpointing[str(k)] = {"arena_bytes": p.program.dag.arena_bytes,
+                                 "dag_steps": len(p.program.steps),
+                                 "recomputed_steps": p.program.recomputed_steps,
+                                 "ms_per_step": round(ms_k, 3)}
+        log(f"c4: checkpoint_every {k}: arena {p.program.dag.arena_bytes / 2**20:.1f} MiB, "
+            f"{ms_k:.1f} ms")
+        p.program.dag.close()
+        del p
+    prob = ParameterizedAmplitude(circ, path, checkpoint_every=args.c4_checkpoint)
+    ms, g = timed(prob)
+    checkpointing[str(args.c4_checkpoint)] = {
+        "arena_bytes": prob.program.dag.arena_bytes, "dag_steps": len(prob.program.steps),
+        "recomputed_steps": prob.program.recomputed_steps, "ms_per_step": round(ms, 3)}
     n_fd = prob.n_params if args.c4_fd <= 0 else min(args.c4_fd, prob.n_params)
     base = prob.theta.copy()
     h = 1e-3
@@ -918,7 +940,9 @@ def run_c4(args):
         "rel_err_vs_central_differences": err,
         "angles_checked": n_fd,
         "method": prob.method,
-        "dag_steps": len(prob.program.steps) if prob.method == "graph" else None,
+        "checkpoint_every": args.c4_checkpoint,
+        "dag_steps": len(prob.program.steps),
+        "checkpointing": checkpointing,
     }, None
 
 
@@ -933,6 +957,9 @@ def main():
                          "second leg unless --single-plan)")
     ap.add_argument("--c4-fd", type=int, default=0,
                     help="c4: angles checked by central differences (0: all)")
+    ap.add_argument("--c4-checkpoint", type=int, default=2,
+                    help="c4: keep every k-th forward intermediate, recompute the others "
+                         "inside the DAG (0: keep all)")
     ap.add_argument("--c3-slices", type=int, default=256,
                     help="c3: slices of the c2 amplitude in the batch")
     ap.add_argument("--batch-log2", type=int, default=None,

@@ -16,12 +16,16 @@ recomputes a dropped intermediate from its nearest kept descendants when it
 needs it.
 
 ``AdjointProgram`` plans the same forward and reverse steps as one
-contraction DAG (``jet_hip_dag_*``) replayed as a hipGraph: every
-intermediate stays in the device arena until its last use (no
-recomputation), the environments that are asked for end in one contiguous
-region read back with a single copy.  It trades memory for launch overhead:
-the per-step host work of ``AdjointPath`` (planning lookups, pool
-allocations, one launch per kernel) is paid once, at capture.
+contraction DAG (``jet_hip_dag_*``) replayed as a hipGraph; the environments
+that are asked for end in one contiguous region read back with a single
+copy.  With ``checkpoint_every = 0`` every forward intermediate stays in the
+device arena until its reverse-pass use; with ``checkpoint_every = k`` only
+every k-th forward intermediate (and the root) is kept, and the DAG itself
+recomputes each dropped one, once, from its nearest kept descendants just
+before the reverse pass first needs it -- the arena then holds the kept
+intermediates plus the recomputed ones still in use.  Either way the
+per-step host work of ``AdjointPath`` (planning lookups, pool allocations,
+one launch per kernel) is paid once, at capture.
 
 ``ParameterizedAmplitude`` applies this to the fSim gates of a grid circuit:
 dA/dtheta_k = sum(E_g * dG_g/dtheta_k), and the loss L = |A|^2 has
@@ -144,67 +148,125 @@ class AdjointPath:
         return out
 
 
+def adjoint_dag_steps(indices: Sequence[Sequence[str]], path: Sequence[Sequence[int]],
+                      wanted: Sequence[int], checkpoint_every: int = 0) -> Dict[str, object]:
+    """Steps of the contraction DAG that gives the root value and the
+    environments of ``wanted`` leaves of a closed network along an SSA path
+    (no device work).
+
+    DAG leaves are the network's n leaves plus one scalar 1 (node n, the
+    root's environment); step s gives DAG node n + 1 + s.  Steps: the forward
+    path, then for each forward step z = (x, y) taken in reverse whose
+    subtree holds a wanted leaf, E_x = contract(E_z, y) and
+    E_y = contract(E_z, x) where needed (the reverse pass of
+    ``AdjointPath.environments``).  With ``checkpoint_every`` k > 0 only
+    forward steps s with s % k == 0 (and the root) are reused by the reverse
+    pass; the value of any other forward node it needs is recomputed by
+    extra steps, appended just before its first use, from its nearest kept
+    descendants, and each dropped node is recomputed at most once.
+
+    Returns a dict: steps (list of DAG node pairs), root_node, env_node
+    ({leaf: DAG node of its environment}), kept (forward node ids reused),
+    recomputed ({forward node: DAG node of its recomputation})."""
+    if checkpoint_every < 0:
+        raise JetError("checkpoint_every must be 0 (keep every intermediate) or positive.")
+    counts: Dict[str, int] = {}
+    for ix in indices:
+        for l in ix:
+            counts[l] = counts.get(l, 0) + 1
+    if any(c != 2 for c in counts.values()):
+        raise JetError("Environments need a closed network: every index on two tensors.")
+    n = len(indices)
+    path = [(int(a), int(b)) for (a, b) in path]
+    if len(path) != n - 1:
+        raise JetError("The path must contract the network to one tensor.")
+    one = n                              # the scalar leaf
+    n_dag_leaves = n + 1
+    steps: List[tuple] = []
+    dag_id: Dict[int, int] = {i: i for i in range(n)}   # forward node -> DAG node
+    for s, (a, b) in enumerate(path):
+        steps.append((dag_id[a], dag_id[b]))
+        dag_id[n + s] = n_dag_leaves + len(steps) - 1
+    root = n + len(path) - 1
+    need: Set[int] = set(int(w) for w in wanted)
+    for s, (a, b) in enumerate(path):
+        if a in need or b in need:
+            need.add(n + s)
+    kept = {n + s for s in range(len(path)) if checkpoint_every == 0 or s % checkpoint_every == 0}
+    kept.add(root)
+    recomputed: Dict[int, int] = {}
+
+    def reused(u: int) -> bool:
+        return u < n or u in kept
+
+    def value(u: int) -> int:
+        """DAG node holding forward node u's value in the reverse pass: the
+        leaf, the kept forward result, or its recomputation (appended here,
+        children first)."""
+        if reused(u):
+            return dag_id[u]
+        todo = [u]
+        while todo:
+            v = todo[-1]
+            if v in recomputed:
+                todo.pop()
+                continue
+            a, b = path[v - n]
+            pending = [c for c in (a, b) if not reused(c) and c not in recomputed]
+            if pending:
+                todo.extend(pending)
+                continue
+            todo.pop()
+            ra = dag_id[a] if reused(a) else recomputed[a]
+            rb = dag_id[b] if reused(b) else recomputed[b]
+            steps.append((ra, rb))
+            recomputed[v] = n_dag_leaves + len(steps) - 1
+        return recomputed[u]
+
+    env: Dict[int, int] = {root: one}
+    env_node: Dict[int, int] = {}
+    for s in range(len(path) - 1, -1, -1):
+        ez = env.pop(n + s, None)
+        if ez is None:
+            continue
+        x, y = path[s]
+        for (u, other) in ((x, y), (y, x)):
+            if u not in need:
+                continue
+            v = value(other)
+            steps.append((ez, v))
+            e = n_dag_leaves + len(steps) - 1
+            if u < n:
+                env_node[u] = e
+            else:
+                env[u] = e
+    missing = [int(w) for w in wanted if int(w) not in env_node]
+    if missing:
+        raise JetError("No environment for leaves %s." % missing)
+    return {"steps": steps, "root_node": dag_id[root], "env_node": env_node, "kept": kept,
+            "recomputed": recomputed}
+
+
 class AdjointProgram:
     """The root value and the environments of ``wanted`` leaves of a closed
-    network along an SSA path, as one planned contraction DAG on the GPU.
-
-    Leaves are the network's leaves plus one scalar 1 (the root's
-    environment).  Steps: the forward path, then for each forward step
-    z = (x, y) taken in reverse whose subtree holds a wanted leaf,
-    E_x = contract(E_z, y) and E_y = contract(E_z, x) where needed (the
-    reverse pass of ``AdjointPath.environments``)."""
+    network along an SSA path, as one planned contraction DAG on the GPU
+    (the steps of ``adjoint_dag_steps``, replayed as a hipGraph)."""
 
     def __init__(self, indices: Sequence[Sequence[str]], shapes, path: Sequence[Sequence[int]],
                  wanted: Sequence[int], dtype=np.complex64, precision: Optional[str] = None,
-                 ctx=None):
-        counts: Dict[str, int] = {}
-        for ix in indices:
-            for l in ix:
-                counts[l] = counts.get(l, 0) + 1
-        if any(c != 2 for c in counts.values()):
-            raise JetError("Environments need a closed network: every index on two tensors.")
-        n = len(indices)
-        path = [(int(a), int(b)) for (a, b) in path]
-        if len(path) != n - 1:
-            raise JetError("The path must contract the network to one tensor.")
-        self.n = n
-        self.one = n                     # the scalar leaf
-        n_dag_leaves = n + 1
-        steps: List[tuple] = []
-        dag_id: Dict[int, int] = {i: i for i in range(n)}   # forward node -> DAG node
-        for s, (a, b) in enumerate(path):
-            steps.append((dag_id[a], dag_id[b]))
-            dag_id[n + s] = n_dag_leaves + len(steps) - 1
-        root = n + len(path) - 1
+                 ctx=None, checkpoint_every: int = 0):
+        plan = adjoint_dag_steps(indices, path, wanted, checkpoint_every)
+        self.n = len(indices)
+        self.one = self.n
         self.wanted = [int(w) for w in wanted]
-        need: Set[int] = set(self.wanted)
-        for s, (a, b) in enumerate(path):
-            if a in need or b in need:
-                need.add(n + s)
-        env: Dict[int, int] = {root: self.one}
-        self.env_node: Dict[int, int] = {}
-        for s in range(len(path) - 1, -1, -1):
-            ez = env.pop(n + s, None)
-            if ez is None:
-                continue
-            x, y = path[s]
-            for (u, other) in ((x, y), (y, x)):
-                if u not in need:
-                    continue
-                steps.append((ez, dag_id[other]))
-                e = n_dag_leaves + len(steps) - 1
-                if u < n:
-                    self.env_node[u] = e
-                else:
-                    env[u] = e
-        missing = [w for w in self.wanted if w not in self.env_node]
-        if missing:
-            raise JetError("No environment for leaves %s." % missing)
-        self.root_node = dag_id[root]
-        self.steps = steps
+        self.checkpoint_every = int(checkpoint_every)
+        self.steps = plan["steps"]
+        self.root_node = plan["root_node"]
+        self.env_node = plan["env_node"]
+        self.recomputed_steps = len(plan["recomputed"])
         self.leaf_labels = [list(ix) for ix in indices]
-        self.dag = DeviceDag(self.leaf_labels + [[]], [list(sh) for sh in shapes] + [[]], steps,
-                             [self.root_node] + [self.env_node[w] for w in self.wanted],
+        self.dag = DeviceDag(self.leaf_labels + [[]], [list(sh) for sh in shapes] + [[]],
+                             self.steps, [self.root_node] + [self.env_node[w] for w in self.wanted],
                              dtype=dtype, precision=precision, ctx=ctx)
         self.ctx = self.dag.ctx
         self.dtype = self.dag.dtype
@@ -233,9 +295,11 @@ class ParameterizedAmplitude:
     def __init__(self, circ: Circuit, path: Sequence[Sequence[int]], dtype=np.complex64,
                  precision: Optional[str] = None, checkpoint_every: int = 4, ctx=None,
                  method: str = "graph"):
-        """method "graph": ``AdjointProgram`` (one captured DAG, every
-        intermediate kept until its last use); "eager": ``AdjointPath``
-        (per-step calls, checkpointed intermediates recomputed)."""
+        """method "graph": ``AdjointProgram`` (one captured DAG; with
+        ``checkpoint_every`` k > 0 it keeps every k-th forward intermediate
+        and recomputes the others inside the DAG, with 0 it keeps them all);
+        "eager": ``AdjointPath`` (per-step calls, checkpointed intermediates
+        recomputed; k >= 1)."""
         if method not in ("graph", "eager"):
             raise JetError(f"Unknown gradient method '{method}'.")
         self.method = method
@@ -270,7 +334,8 @@ class ParameterizedAmplitude:
             return
         self.adjoint = None
         self.program = AdjointProgram(net.indices, net.shapes, self.path, self.param_leaf,
-                                      dtype=self.dtype, precision=precision, ctx=self.ctx)
+                                      dtype=self.dtype, precision=precision, ctx=self.ctx,
+                                      checkpoint_every=checkpoint_every)
         # Every fSim gate in one device buffer (one upload per angle update).
         item = self.dtype.itemsize
         is_param = set(self.param_leaf)

@@ -4,12 +4,13 @@
   reference gradient (oracle.environments, itself pinned against exact
   statevector derivatives), for every fSim angle, with both methods: the
   captured contraction DAG ("graph") and per-step calls ("eager");
-  checkpointed recomputation gives the same gradient as keeping every
-  intermediate.
+  checkpointed recomputation (eager, and inside the DAG) gives the same
+  gradient as keeping every intermediate.
 * A triangle network run as a contraction DAG: every kept value against
   numpy.
-* 6x6 grid, depth 10, ~300 angles ~ U(0, 2 pi): the torch.autograd.Function
-  gradient against central finite differences (h = 1e-3, complex128 on FP64
+* 6x6 grid, depth 10, ~300 angles ~ U(0, 2 pi), every 4th forward
+  intermediate kept and the others recomputed inside the DAG: the
+  torch.autograd.Function gradient against central finite differences (h = 1e-3, complex128 on FP64
   MFMA) for every angle, normwise relative error <= 1e-4, and against the
   CPU reference gradient.
 """
@@ -75,11 +76,19 @@ def test_gradient_4x4_against_cpu_reference():
 
     say("gradient 4x4 d8, contraction DAG")
     pg = ParameterizedAmplitude(circ, path)
-    assert pg.method == "graph"
+    assert pg.method == "graph" and pg.program.recomputed_steps > 0
     loss_g, g_g = pg.loss_and_gradient()
     assert abs(pg.amplitude - a_ref) / abs(a_ref) < 1e-5
     assert _rel(g_g, g_ref) < 1e-4, _rel(g_g, g_ref)
     assert _rel(g_g, g) < 1e-5, _rel(g_g, g)
+    # The DAG without checkpoints: a recomputed node is the same contraction
+    # of the same values, so the gradient is the same to the bit.
+    say("gradient 4x4 d8, contraction DAG keeping every intermediate")
+    p0 = ParameterizedAmplitude(circ, path, checkpoint_every=0)
+    assert p0.program.recomputed_steps == 0
+    assert p0.program.dag.arena_bytes > pg.program.dag.arena_bytes
+    loss_0, g_0 = p0.loss_and_gradient()
+    assert loss_0 == loss_g and np.array_equal(g_0, g_g)
     # New angles: one upload, the same captured graph.
     theta = pg.theta + 0.1
     loss_e, g_e = prob.loss_and_gradient(theta)
@@ -128,8 +137,10 @@ def test_gradient_6x6_torch_against_finite_differences():
     torch = pytest.importorskip("torch")
     circ, net, path = _setup(6, 6, 10, seed=2026)
     prob = ParameterizedAmplitude(circ, path, checkpoint_every=4)
+    assert prob.method == "graph" and prob.program.recomputed_steps > 0
     theta = torch.tensor(prob.theta, dtype=torch.float64, requires_grad=True)
-    say(f"gradient 6x6 d10: {prob.n_params} angles, {len(path)} steps")
+    say(f"gradient 6x6 d10: {prob.n_params} angles, {len(path)} steps, "
+        f"{prob.program.recomputed_steps} recomputed in the DAG")
     loss = AmplitudeLoss.apply(theta, prob)
     loss.backward()
     g = theta.grad.numpy().copy()
