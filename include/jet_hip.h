@@ -154,6 +154,22 @@ int jet_hip_contract(jet_hip_plan_t plan, const void *a, const void *b,
                      void *c, void *workspace, size_t workspace_bytes,
                      void *stream);
 
+/* jet_hip_contract without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * under hipGraph stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The same kernels as jet_hip_contract run with a fixed
+ * per-device scratch array of JET_HIP_NOALLOC_SCRATCH_BYTES as the
+ * workspace (the library holds it on each GPU it runs on until the process
+ * ends); no hipMalloc, hipFree or other allocating call is made.  A plan
+ * whose jet_hip_plan_workspace_size exceeds that limit is rejected with
+ * JET_HIP_ERROR_INVALID_VALUE before anything is queued.  All calls on one
+ * GPU share its scratch array: calls whose executions overlap on that GPU
+ * (concurrent streams, or host threads without ordering) race and must be
+ * serialized by the caller. */
+#define JET_HIP_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int jet_hip_contract_noalloc(jet_hip_plan_t plan, const void *a,
+                             const void *b, void *c, void *stream);
+
 /* ---- elementwise (replaces cutensorElementwiseBinary in
  *      CudaTensor::AddTensors, include/jet/CudaTensor.hpp:38-137, and
  *      Tensor::AddTensors, include/jet/Tensor.hpp:412-454)

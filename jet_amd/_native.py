@@ -36,7 +36,7 @@ EXPORTS = [
     "jet_hip_memcpy", "jet_hip_memset", "jet_hip_stream_synchronize",
     "jet_hip_device_synchronize", "jet_hip_permute", "jet_hip_contraction_plan_create",
     "jet_hip_plan_workspace_size", "jet_hip_plan_info", "jet_hip_plan_destroy",
-    "jet_hip_contract", "jet_hip_add", "jet_hip_slice", "jet_hip_fill_random",
+    "jet_hip_contract", "jet_hip_contract_noalloc", "jet_hip_add", "jet_hip_slice", "jet_hip_fill_random",
     "jet_hip_reduce_compensated", "jet_hip_convert", "jet_hip_event_create",
     "jet_hip_event_record", "jet_hip_event_elapsed_ms", "jet_hip_event_destroy",
     "jet_hip_profiler_enable", "jet_hip_profiler_reset", "jet_hip_profiler_read",
@@ -88,6 +88,7 @@ def _declare(L):
                               ctypes.POINTER(ctypes.c_int)],
         "jet_hip_plan_destroy": [_vp],
         "jet_hip_contract": [_vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
+        "jet_hip_contract_noalloc": [_vp, _vp, _vp, _vp, _vp],
         "jet_hip_add": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i64p, _i32p, _vp,
                         _vp, _vp, _vp],
         "jet_hip_slice": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int,

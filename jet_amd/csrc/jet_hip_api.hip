@@ -254,6 +254,37 @@ void free_graph(jet_hip_network_s *net)
 
 } // namespace
 
+// Workspace of jet_hip_contract_noalloc: one fixed device array per GPU that
+// the library runs on, held until the process ends.  Host code never uses
+// the array's name as an address (on the host it is only a stand-in for the
+// symbol): scratch_setup looks up each device's own copy once, with that
+// device current, and every later use goes through that pointer.
+static_assert(JET_HIP_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch size");
+static __device__ __align__(256) char scratch[JET_HIP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+char *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of this device's scratch array (the device must be
+// current); looked up on the first call per device.
+char *scratch_setup(int device)
+{
+    JH_REQUIRE(device >= 0 && device < kMaxScratchDevices, "Invalid device ordinal.");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        JH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        JH_REQUIRE(p != nullptr, "No device address for the no-allocation scratch array.");
+        g_scratch_ptr[device] = static_cast<char *>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
+} // namespace
+
 extern "C" {
 
 int jet_hip_get_version(int *major, int *minor, int *patch)
@@ -286,6 +317,14 @@ int jet_hip_ctx_create(int device, jet_hip_ctx_t *ctx)
         if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
             uint64_t threshold = UINT64_MAX;
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+        }
+        try {
+            (void)scratch_setup(device);
+        }
+        catch (...) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            throw;
         }
         *ctx = c;
     });
@@ -486,6 +525,29 @@ int jet_hip_contract(jet_hip_plan_t plan, const void *a, const void *b, void *c,
             throw;
         }
         if (pooled) JH_HIP(hipFreeAsync(ws, s));
+    });
+}
+
+int jet_hip_contract_noalloc(jet_hip_plan_t plan, const void *a, const void *b, void *c,
+                             void *stream)
+{
+    return guarded([&] {
+        JH_REQUIRE(plan != nullptr, "Null plan.");
+        jet_hip_ctx_s *ctx = plan->ctx;
+        set_device(ctx);
+        const ContractionDesc &d = plan->d;
+        JH_REQUIRE(a && b && c, "Null tensor data.");
+        if (d.workspace > JET_HIP_NOALLOC_SCRATCH_BYTES) {
+            const std::string msg = "the contraction needs " + std::to_string(d.workspace) +
+                                    " bytes of workspace; jet_hip_contract_noalloc holds " +
+                                    std::to_string(size_t(JET_HIP_NOALLOC_SCRATCH_BYTES)) + ".";
+            JH_FAIL(JET_HIP_ERROR_INVALID_VALUE, msg.c_str());
+        }
+        // Same kernels as jet_hip_contract, with the device's scratch array
+        // as the workspace.  Every part of the workspace is written before it
+        // is read within one contraction, so nothing needs clearing.
+        char *ws = d.workspace > 0 ? scratch_setup(ctx->device) : nullptr;
+        run_contraction(d, a, b, c, ws, pick_stream(ctx, stream));
     });
 }
 
