@@ -12,7 +12,7 @@ Writes tests/golden/c2_amplitude_fp64.json (the amplitude, plus the
 compensated sum of the CPU-checked slices' GPU values as a cross-check).
 About two minutes on one MI355X.
 
-Usage: python tests/golden/make_c2_amplitude.py
+Usage: python tests/golden/make_c2_amplitude.py [--out PATH]
 """
 import json
 import os
@@ -32,6 +32,9 @@ OUT = os.path.join(HERE, "c2_amplitude_fp64.json")
 
 
 def main():
+    out = OUT
+    if "--out" in sys.argv:
+        out = sys.argv[sys.argv.index("--out") + 1]
     with open(os.path.join(HERE, "c2_slices_fp64.json")) as f:
         fixture = json.load(f)
     with open(os.path.join(ROOT, fixture["plan"])) as f:
@@ -63,9 +66,9 @@ def main():
            "checked_slices_rel_err": err,
            "made_by": "tests/golden/make_c2_amplitude.py (complex128, FP64 MFMA, "
                       "compensated sum over all slices)"}
-    with open(OUT, "w") as f:
+    with open(out, "w") as f:
         json.dump(doc, f, indent=1)
-    print(f"amplitude {amp!r}; wrote {OUT}", flush=True)
+    print(f"amplitude {amp!r}; wrote {out}", flush=True)
 
 
 if __name__ == "__main__":

@@ -7,7 +7,9 @@ order UnravelIndex over the 12 sliced indices, TensorNetwork.hpp:230).  Both
 committed plans run those slices through SlicedContraction (4 sliced indices
 batched, the other 8 looped, so slices past 15 exercise the loop indices) in
 split-FP16, FP32 MFMA and complex128 on FP64 MFMA.  Tolerances: 1e-5
-normwise over the 12 slices for complex64, 1e-10 for complex128.
+normwise over the 12 slices for complex64, 1e-10 for complex128.  The whole
+4096-slice amplitude in split-FP16 is checked against the complex128
+amplitude of tests/golden/c2_amplitude_fp64.json (1e-5 relative).
 """
 import json
 import os
@@ -58,3 +60,29 @@ def test_c2_slices_match_cpu_oracle(ctx, c2, plan_name, prec, dtype, tol):
     err = float(np.linalg.norm(got - want) / np.linalg.norm(want))
     say(f"  normwise error {err:.2e}")
     assert err < tol, err
+
+
+def test_c2_whole_amplitude_split_fp16(ctx, c2):
+    """All 4096 slices along the headline plan (256 per execution, the bench's
+    setting) in split-FP16, summed in FP64 by the compensated reduction,
+    against tests/golden/c2_amplitude_fp64.json (complex128 on FP64 MFMA over
+    every slice, accepted only after its slices matched the CPU oracle's to
+    1e-10; tests/golden/make_c2_amplitude.py): relative error <= 1e-5."""
+    _, net, _ = c2
+    with open(os.path.join(ROOT, "tests", "golden", "c2_amplitude_fp64.json")) as f:
+        ref = json.load(f)
+    with open(os.path.join(ROOT, ref["plan"])) as f:
+        plan = json.load(f)
+    assert ref["slices"] == 4096 and ref["checked_slices_rel_err"] < 1e-10
+    sliced = plan["sliced"]
+    sc = SlicedContraction(net.indices, net.shapes, list(net.data),
+                           [tuple(p) for p in plan["path"]], sliced, batch=sliced[-8:],
+                           dtype=np.complex64, precision="f16x3", ctx=ctx, use_graph=False)
+    say(f"c2 whole amplitude split-FP16: {sc.n_loop} executions of {sc.n_batch} slices")
+    sc.run()
+    amp = sc.reduce()
+    sc.net.close()
+    want = complex(*ref["amplitude"])
+    err = abs(amp - want) / abs(want)
+    say(f"  amplitude {amp:.6e}, relative error {err:.2e}")
+    assert err <= 1e-5, err
