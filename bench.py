@@ -352,6 +352,33 @@ C2_BATCH_LOG2 = {"refined": 8, "min-fill": 7}
 C2_PROFILE = {"refined": "c2_sliced_amplitude.md", "min-fill": "c2_min_fill_path.md"}
 
 
+def hbm_copy_ceiling(ctx, gib=2, reps=10):
+    """The library's own streaming copy (the identity permutation) on a
+    `gib` GiB complex64 tensor: read + write bytes per second, the practical
+    HBM ceiling that permutations and splits are judged against."""
+    from jet_amd import _native as nat
+    n = (gib << 30) // 8
+    src, dst = nat.DeviceBuffer(n * 8, ctx), nat.DeviceBuffer(n * 8, ctx)
+    nat.check(nat.lib().jet_hip_fill_random(ctx.handle, nat.C64, n, 7, 1, src.ptr, None))
+    modes, ext = nat.i32([0]), nat.i64([n])
+
+    def copy():
+        nat.check(nat.lib().jet_hip_permute(ctx.handle, nat.C64, nat.ROW_MAJOR, 1, modes, ext,
+                                            modes, src.ptr, dst.ptr, None))
+    copy()
+    ctx.synchronize()
+    e0, e1 = nat.Event(ctx), nat.Event(ctx)
+    e0.record()
+    for _ in range(reps):
+        copy()
+    e1.record()
+    ctx.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    src.free()
+    dst.free()
+    return 2.0 * n * 8 / (ms * 1e-3)
+
+
 def c2_problem(plan_name="min-fill"):
     from jet_amd import circuits, paths
     if plan_name == "min-fill":
@@ -453,6 +480,14 @@ def run_c2(args):
         f"{sc16.net.arena_bytes / 2**30:.1f} GiB, workspace {sc16.net.workspace_bytes / 2**30:.1f} GiB, "
         f"{sc16.n_loop} executions of {n_batch} slices per amplitude")
     ms16, prof16, sum16, done16 = timed(sc16, steps, warm, True)
+    if args.quick:
+        # Profiler runs: the split leg's executions and the 2 GiB streaming
+        # copy of known size that calibrates the HBM traffic counters.
+        copy_bps = hbm_copy_ceiling(ctx)
+        log(f"c2 quick: {ms16 / steps:.1f} ms/execution, copy {copy_bps / 1e12:.2f} TB/s")
+        return {"metric": METRIC, "value": round(steps * n_batch / (ms16 * 1e-3), 3),
+                "unit": "slices/s", "ms_per_step": round(ms16 / steps, 3),
+                "kernels_split": prof16, "copy_ceiling_gbps": round(copy_bps / 1e9, 1)}, None
     if done16 < n_slices:
         # The whole amplitude needs every execution: run the rest untimed.
         sc16.run(range(warm + steps, sc16.n_loop))
@@ -522,6 +557,13 @@ def run_c2(args):
     tflops32 = exec_flops * steps32 / (ms32 * 1e-3)
     sps16 = slices16 / (ms16 * 1e-3)
     sps32 = steps32 * n_batch / (ms32 * 1e-3)
+    copy_bps = hbm_copy_ceiling(ctx)
+    perm = prof16.get("permute_bits")
+    perm_bps = perm["bytes"] / (perm["ms"] * 1e-3) if perm and perm["ms"] > 0 else None
+    split = prof16.get("split_f16")
+    split_bps = split["bytes"] / (split["ms"] * 1e-3) if split and split["ms"] > 0 else None
+    log(f"c2: copy ceiling {copy_bps / 1e12:.2f} TB/s; permutations "
+        f"{(perm_bps or 0) / 1e12:.2f} TB/s; splits {(split_bps or 0) / 1e12:.2f} TB/s")
     dom = max(prof16.items(), key=lambda kv: kv[1]["ms"])
     dname, dk = dom
     bound = "mfma" if dk["flops"] > 0 else "hbm"
@@ -572,6 +614,15 @@ def run_c2(args):
         "full_amplitude": (c2_full_amplitude(ctx, net, plan, batch, SPLIT_MODES[args.split][1])
                            if args.full_amplitude else None),
         "fp64_mfma": {"ms_per_step": round(ms64, 3), "slices_per_step": n_ref},
+        "hbm": {"copy_ceiling_gbps": round(copy_bps / 1e9, 1),
+                "copy_frac_of_8tbps": round(copy_bps / PEAK["hbm"], 4),
+                "permute_gbps": None if perm_bps is None else round(perm_bps / 1e9, 1),
+                "permute_frac_of_copy": None if perm_bps is None else round(perm_bps / copy_bps, 4),
+                "split_gbps": None if split_bps is None else round(split_bps / 1e9, 1),
+                "split_frac_of_copy": None if split_bps is None else round(split_bps / copy_bps, 4),
+                "note": "read + write bytes per second of the library's own streaming copy "
+                        "(the identity permutation, 2 GiB) and of the execution's "
+                        "permutation / split passes (algorithmic bytes)"},
         "kernels_split": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
                           for k, v in prof16.items()},
         "kernels_fp32": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)}
@@ -983,7 +1034,8 @@ def main():
                          "captured hipGraph (for kernel tracers)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--quick", action="store_true",
-                    help="only the timed split-BF16 region (for profiler runs)")
+                    help="only the timed split region and a 2 GiB streaming copy "
+                         "(for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the slice "
@@ -1004,6 +1056,9 @@ def main():
         args.no_cpu_baseline = True
     else:
         result, host = run_c2(args)
+        if args.quick:
+            print(json.dumps(result), flush=True)
+            return
     # Every host core this process may run on.
     threads = args.cpu_threads or len(os.sched_getaffinity(0))
     if not args.no_cpu_baseline:

@@ -170,6 +170,50 @@ int jet_hip_contract(jet_hip_plan_t plan, const void *a, const void *b,
 int jet_hip_contract_noalloc(jet_hip_plan_t plan, const void *a,
                              const void *b, void *c, void *stream);
 
+/* ---- GEMM kernel autotuning (replaces cuTENSOR's kernel selection behind
+ *      cutensorInitContractionFind(CUTENSOR_ALGO_DEFAULT) and
+ *      cutensorInitContractionPlan, include/jet/CudaTensor.hpp:616-640).
+ *
+ * Kernel ids (jet_hip_plan_info's gemm_kernel): 0 VALU per output, 1 VALU
+ * block reduction, 2 FP32 MFMA, 3 FP64 MFMA, 4 split-BF16 3M, 5 split-BF16
+ * 4M, 6 split-K with FP64 partials, 7 split-FP16 4M.  Variants: kernel 2
+ * 1 = 128x128 / 2 = 64x128 / 3 = 128x64 block tiles; kernels 5 and 7
+ * 1 = 128x128 / 2 = 128x192 block tiles / 3 = 128x128 tiles walked by a
+ * persistent grid of one work-group per CU; kernel 0 1 = one thread per output
+ * / 2 = short-K column blocks / 3 = short-K row threads (K <= 16); every
+ * other kernel 1.
+ *
+ * The library holds a table of (precision mode, log2 M, log2 N, log2 K,
+ * log2 batch) -> (kernel, variant), measured on gfx950 and compiled in.
+ * Every plan (contraction, network, DAG) takes its GEMM from the entry of
+ * its shape, else from the nearest entry of its mode whose choice can run
+ * the shape, else from a built-in rule.  Entries are 7 int32 each:
+ * {mode, log2 M, log2 N, log2 K, log2 batch, kernel, variant}, mode 0 FP32,
+ * 1 split-BF16 3M, 2 FP64, 3 split-BF16 4M, 4 split-FP16 4M.            */
+
+/* The kernel and variant a plan of this shape would use; source: 0 the
+ * built-in rule, 1 the shape's own table entry, 2 the nearest entry.
+ * Host-only (no device needed). */
+int jet_hip_gemm_choice(jet_hip_dtype_t dtype, jet_hip_precision_t precision,
+                        int64_t m, int64_t n, int64_t k, int64_t batch,
+                        int *kernel, int *variant, int *source);
+/* Replace the process-wide table with n entries (n = 0: built-in rule only;
+ * n < 0: restore the compiled-in table); plans created afterwards use it. */
+int jet_hip_tuning_set(int n, const int32_t *entries);
+/* The current table: *n entries, the first `capacity` copied out. */
+int jet_hip_tuning_get(int capacity, int32_t *entries, int *n);
+/* Time every kernel and variant that can run the batched NT product
+ * C[b][m][n] = sum_k A[b][m][k] B[b][n][k] of this shape and precision
+ * (random operands in pool memory; the split modes' operand passes are part
+ * of the time): mean ms over `reps` launches after one warm-up, per
+ * candidate, in kernels / variants / ms (up to capacity), and the number of
+ * candidates in *count.  Runs on the context stream and waits for it. */
+int jet_hip_autotune_gemm(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
+                          jet_hip_precision_t precision, int64_t m, int64_t n,
+                          int64_t k, int64_t batch, int reps, int capacity,
+                          int32_t *kernels, int32_t *variants, float *ms,
+                          int *count);
+
 /* ---- elementwise (replaces cutensorElementwiseBinary in
  *      CudaTensor::AddTensors, include/jet/CudaTensor.hpp:38-137, and
  *      Tensor::AddTensors, include/jet/Tensor.hpp:412-454)

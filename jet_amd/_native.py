@@ -44,7 +44,8 @@ EXPORTS = [
     "jet_hip_network_step_info", "jet_hip_network_execute", "jet_hip_network_capture",
     "jet_hip_network_launch", "jet_hip_network_destroy", "jet_hip_batch_capture",
     "jet_hip_batch_launch", "jet_hip_batch_destroy", "jet_hip_dag_create", "jet_hip_dag_estimate",
-    "jet_hip_dag_value",
+    "jet_hip_dag_value", "jet_hip_gemm_choice", "jet_hip_tuning_set", "jet_hip_tuning_get",
+    "jet_hip_autotune_gemm",
 ]
 
 
@@ -89,6 +90,15 @@ def _declare(L):
         "jet_hip_plan_destroy": [_vp],
         "jet_hip_contract": [_vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
         "jet_hip_contract_noalloc": [_vp, _vp, _vp, _vp, _vp],
+        "jet_hip_gemm_choice": [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+        "jet_hip_tuning_set": [ctypes.c_int, _i32p],
+        "jet_hip_tuning_get": [ctypes.c_int, _i32p, ctypes.POINTER(ctypes.c_int)],
+        "jet_hip_autotune_gemm": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                  ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                  _i32p, _i32p, ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(ctypes.c_int)],
         "jet_hip_add": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i64p, _i32p, _vp,
                         _vp, _vp, _vp],
         "jet_hip_slice": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int,

@@ -1,5 +1,6 @@
 // contract_plan.cpp -- host-side planning of one pairwise contraction.
 #include "contract_plan.hpp"
+#include "tuning.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -112,6 +113,50 @@ bool plan_split_gather(const std::vector<int32_t> &src, const std::vector<int64_
         }
     }
     return j == g.tb;
+}
+
+int gemm_variants(int kernel)
+{
+    switch (kernel) {
+    case kGemmF32Mfma:
+        return 3;
+    case kGemmBf16x4m:
+    case kGemmF16x4m:
+        return 3;
+    case kGemmSmall:
+        return 3;
+    default:
+        return 1;
+    }
+}
+
+bool gemm_choice_valid(int mode, int64_t M, int64_t N, int64_t K, int64_t batch, GemmChoice c)
+{
+    if (M < 1 || N < 1 || K < 1 || batch < 1) return false;
+    if (c.variant < 0 || c.variant > gemm_variants(c.kernel)) return false;
+    const bool c128 = mode == 2;
+    const int64_t outputs = batch * M * N;
+    switch (c.kernel) {
+    case kGemmSmall:
+        if (c.variant == 2 || c.variant == 3) return K <= 16;
+        return true;
+    case kGemmSmallRed:
+        return outputs <= 0x7fffffff;
+    case kGemmSplitK:
+        return outputs <= 0x7fffffff && ceil_div(K, splitk_chunk(K)) <= 65535;
+    case kGemmF32Mfma:
+        return !c128 && batch <= 65535 && ceil_div(M, 64) * ceil_div(N, 64) <= 0x7fffffff;
+    case kGemmF64Mfma:
+        return c128 && batch <= 65535 && ceil_div(M, 128) * ceil_div(N, 64) <= 0x7fffffff;
+    case kGemmBf16x3:
+        return mode == 1 && batch <= 65535 && ceil_div(M, 128) * ceil_div(N, 128) <= 0x7fffffff;
+    case kGemmBf16x4m:
+        return mode == 3 && batch <= 65535 && ceil_div(M, 128) * ceil_div(N, 128) <= 0x7fffffff;
+    case kGemmF16x4m:
+        return mode == 4 && batch <= 65535 && ceil_div(M, 128) * ceil_div(N, 128) <= 0x7fffffff;
+    default:
+        return false;
+    }
 }
 
 int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
@@ -252,7 +297,9 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
         d.need_c = d.pc.kind != PermutePlan::kCopy;
     }
 
-    d.gemm_kernel = choose_gemm_kernel(mode, d.M, d.N, d.K, d.batch);
+    const GemmChoice choice = tuned_gemm_choice(mode, d.M, d.N, d.K, d.batch, nullptr);
+    d.gemm_kernel = choice.kernel;
+    d.gemm_variant = choice.variant;
     d.flops = 8.0 * double(d.batch) * double(d.M) * double(d.N) * double(d.K);
     if (d.gemm_kernel == kGemmF16x4m) {
         // fold a permutation into the split pass when the rows need no
@@ -264,8 +311,14 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
             d.fuse_b = plan_split_gather(B, EB, cat(batch, right, common), d.K, d.gb);
     }
 
+    layout_workspace(d);
+    return d;
+}
+
+void layout_workspace(ContractionDesc &d)
+{
     size_t off = 0;
-    const size_t eb = (size_t)elem_bytes;
+    const size_t eb = (size_t)d.elem_bytes;
     if (d.need_a && !d.fuse_a) {
         d.off_at = off;
         off = align256(off + eb * (size_t)d.size_a);
@@ -305,7 +358,6 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
         off = align256(off + 16 * (size_t)splitk_partials(d.batch * d.M * d.N, d.K));
     }
     d.workspace = off;
-    return d;
 }
 
 } // namespace jet_hip

@@ -36,6 +36,7 @@ struct ContractionDesc {
     SplitGather ga, gb;
     PermutePlan pa, pb, pc;
     int gemm_kernel = kGemmSmall;
+    int gemm_variant = 0; // tile variant of gemm_kernel (GemmChoice)
     // Workspace layout (bytes, 256-aligned offsets).
     size_t off_at = 0, off_bt = 0, off_ct = 0, off_sa = 0, off_sb = 0, off_partials = 0;
     // split-FP16 row maxima and scales of A and B (batch * padded rows each)
@@ -51,6 +52,10 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
                                  const int64_t *ext_a, int rb,
                                  const int32_t *modes_b, const int64_t *ext_b,
                                  int rc, const int32_t *modes_c);
+
+// Workspace offsets and size of a description whose shape, permutation
+// flags and GEMM kernel are set.
+void layout_workspace(ContractionDesc &d);
 
 // Gather plan for the split-FP16 operand [batch][rows][K] (target, K the
 // product of the last modes' extents) of a source tensor (row-major modes

@@ -1311,6 +1311,187 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_wide_kernel(
     }
 }
 
+// Persistent form of the same product for GEMMs with short k loops and many
+// output tiles (the skinny M or N = 128-256 steps of sliced circuits): a
+// grid of one work-group per CU walks the output tiles (t, t + gridDim.x,
+// ...; gridDim.x a multiple of the XCD count keeps every tile of a
+// work-group on its XCD).  After a tile's last k step the LDS buffers are
+// free: the work-group issues the first k tile of its next output tile into
+// LDS, then writes the finished tile straight from the accumulators (each
+// 16-lane group stores a 128-byte row segment), so the stores drain while
+// the next tile's loads are in flight instead of one after the other.  k
+// loop as in cgemm_x4m_wide_kernel (next k tile issued in two halves over
+// the first two A fragments).
+template <class E, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_persist_kernel(
+    SplitOperand a, SplitOperand b, float2 *__restrict__ Cout, int64_t M,
+    int64_t N, int64_t ldc, int64_t sC, int a_bcast, int b_bcast, int64_t batch)
+{
+    using vec = typename Mfma16<E>::vec;
+    typedef __attribute__((address_space(3))) void lds_void;
+    constexpr int NW = WM * WN;
+    constexpr int KT = 32;
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    constexpr int A_IN = BM * 16 / (64 * NW);
+    constexpr int B_IN = BN * 16 / (64 * NW);
+    constexpr int L = A_IN + B_IN;
+    static_assert(A_IN * 64 * NW == BM * 16 && B_IN * 64 * NW == BN * 16, "tile");
+    static_assert(FM >= 2, "loads are issued over the first two A fragments");
+    __shared__ vec As[2][BM * 16];
+    __shared__ vec Bs[2][BN * 16];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int64_t tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+    const int64_t tiles = tiles_m * tiles_n, total = tiles * batch;
+    const int64_t nkt_a = a.k_p / KT, nkt_b = b.k_p / KT;
+    const int nk = int(a.k_p / KT);
+    int64_t t = blockIdx.x;
+    if (t >= total) return;
+
+    // The tile being loaded: coordinates, operand bases, per-lane offsets.
+    int64_t m0 = 0, n0 = 0, bz = 0;
+    const E *Ab = nullptr, *Bb = nullptr;
+    int64_t off[L];
+    auto locate = [&](int64_t tile) {
+        bz = tile / tiles;
+        int64_t tm, tn;
+        tile_coords(tile % tiles, tiles_m, tiles_n, tm, tn);
+        m0 = tm * BM;
+        n0 = tn * BN;
+        Ab = static_cast<const E *>(a.base) + (a_bcast ? 0 : bz * a.rows_p * a.k_p * 4);
+        Bb = static_cast<const E *>(b.base) + (b_bcast ? 0 : bz * b.rows_p * b.k_p * 4);
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const bool isa = i < A_IN;
+            const int p = (wave * (isa ? A_IN : B_IN) + (isa ? i : i - A_IN)) * 64 + lane;
+            const int r = p >> 4, slot = p & 15;
+            const int64_t rows = isa ? a.rows_p : b.rows_p;
+            int64_t m = (isa ? m0 : n0) + r;
+            if (m >= rows) m = rows - 1;
+            off[i] = (m * (isa ? nkt_a : nkt_b) * 16 + (slot ^ (r & 15))) * 8;
+        }
+    };
+    auto issue = [&](int lo, int hi, int kt, int s) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            if (i < lo || i >= hi) continue;
+            if (i < A_IN)
+                __builtin_amdgcn_global_load_lds(
+                    (const void *)(Ab + off[i] + int64_t(kt) * 128),
+                    (lds_void *)(&As[s][(wave * A_IN + i) * 64]), 16, 0, 0);
+            else
+                __builtin_amdgcn_global_load_lds(
+                    (const void *)(Bb + off[i] + int64_t(kt) * 128),
+                    (lds_void *)(&Bs[s][(wave * B_IN + i - A_IN) * 64]), 16, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
+    f32x4 cr[FM][FN], ci[FM][FN];
+    const int q = lane >> 4, j = lane & 15;
+    auto compute = [&](int s, int kt2, int s2) {
+        const vec *as = As[s], *bs = Bs[s];
+        vec brh[FN], brl[FN], bih[FN], bil[FN];
+#pragma unroll
+        for (int fn = 0; fn < FN; fn++) {
+            const int r = wn * WTN + fn * 16 + j;
+            brh[fn] = bs[r * 16 + ((0 * 4 + q) ^ j)];
+            brl[fn] = bs[r * 16 + ((1 * 4 + q) ^ j)];
+            bih[fn] = bs[r * 16 + ((2 * 4 + q) ^ j)];
+            bil[fn] = bs[r * 16 + ((3 * 4 + q) ^ j)];
+        }
+#pragma unroll
+        for (int fm = 0; fm < FM; fm++) {
+            constexpr int H = (L + 1) / 2;
+            if (fm < 2 && kt2 >= 0) issue(fm * H, fm * H + H, kt2, s2);
+            const int r = wm * WTM + fm * 16 + j;
+            const vec arh = as[r * 16 + ((0 * 4 + q) ^ j)];
+            const vec arl = as[r * 16 + ((1 * 4 + q) ^ j)];
+            const vec aih = as[r * 16 + ((2 * 4 + q) ^ j)];
+            const vec ail = as[r * 16 + ((3 * 4 + q) ^ j)];
+            const vec naih = neg16x8(aih), nail = neg16x8(ail);
+#pragma unroll
+            for (int fn = 0; fn < FN; fn++) {
+                f32x4 x = cr[fm][fn], y = ci[fm][fn];
+                x = Mfma16<E>::mma(arh, brh[fn], x);
+                y = Mfma16<E>::mma(arh, bih[fn], y);
+                x = Mfma16<E>::mma(arh, brl[fn], x);
+                y = Mfma16<E>::mma(arh, bil[fn], y);
+                x = Mfma16<E>::mma(arl, brh[fn], x);
+                y = Mfma16<E>::mma(arl, bih[fn], y);
+                x = Mfma16<E>::mma(naih, bih[fn], x);
+                y = Mfma16<E>::mma(aih, brh[fn], y);
+                x = Mfma16<E>::mma(naih, bil[fn], x);
+                y = Mfma16<E>::mma(aih, brl[fn], y);
+                x = Mfma16<E>::mma(nail, bih[fn], x);
+                y = Mfma16<E>::mma(ail, brh[fn], y);
+                cr[fm][fn] = x;
+                ci[fm][fn] = y;
+            }
+        }
+    };
+
+    constexpr int kWaitLoads = 0x0f70; // vmcnt(0)
+    locate(t);
+    issue(0, L, 0, 0);
+    for (;;) {
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int jj = 0; jj < FN; jj++) {
+                cr[i][jj] = f32x4{0, 0, 0, 0};
+                ci[i][jj] = f32x4{0, 0, 0, 0};
+            }
+        __builtin_amdgcn_s_waitcnt(kWaitLoads);
+        __syncthreads();
+        for (int kt = 0; kt < nk; kt += 2) {
+            compute(0, kt + 1 < nk ? kt + 1 : -1, 1);
+            __builtin_amdgcn_s_waitcnt(kWaitLoads);
+            __syncthreads();
+            if (kt + 1 >= nk) break;
+            compute(1, kt + 2 < nk ? kt + 2 : -1, 0);
+            __builtin_amdgcn_s_waitcnt(kWaitLoads);
+            __syncthreads();
+        }
+        // Every wave is past its last LDS read of this tile: start the next
+        // tile's first k step, then store this one.
+        const int64_t cm0 = m0, cn0 = n0, cbz = bz;
+        const int64_t next = t + gridDim.x;
+        const bool more = next < total;
+        if (more) {
+            locate(next);
+            issue(0, L, 0, 0);
+        }
+        float2 *C = Cout + cbz * sC;
+        const float *sa = a.scale ? a.scale + (a_bcast ? 0 : cbz) * a.rows_p : nullptr;
+        const float *sb = b.scale ? b.scale + (b_bcast ? 0 : cbz) * b.rows_p : nullptr;
+#pragma unroll
+        for (int fm = 0; fm < FM; fm++) {
+#pragma unroll
+            for (int fn = 0; fn < FN; fn++) {
+                const int64_t col = cn0 + wn * WTN + fn * 16 + j;
+                const float fb = (sb && col < N) ? sb[col] : 1.f;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int64_t row = cm0 + wm * WTM + fm * 16 + 4 * q + r;
+                    if (row < M && col < N) {
+                        // Power-of-two factors, applied one at a time so
+                        // that their product cannot leave the float range.
+                        const float fa = sa ? sa[row] : 1.f;
+                        C[row * ldc + col] =
+                            float2{cr[fm][fn][r] * fa * fb, ci[fm][fn][r] * fa * fb};
+                    }
+                }
+            }
+        }
+        if (!more) break;
+        t = next;
+    }
+}
+
 double gemm_flops(const GemmArgs &g)
 {
     return 8.0 * double(g.batch) * double(g.M) * double(g.N) * double(g.K);
@@ -1427,14 +1608,17 @@ void launch_valu(int kernel, const GemmArgs &g, hipStream_t stream)
         hipLaunchKernelGGL(cgemm_smallred_kernel<R>, dim3((unsigned)total), dim3(256), 0,
                            stream, g);
     }
-    else if (g.K <= kShortK && g.N >= 64 && total >= (int64_t(1) << 16)) {
+    else if (g.variant == 2 ||
+             (g.variant == 0 && g.K <= kShortK && g.N >= 64 && total >= (int64_t(1) << 16))) {
+        JH_REQUIRE(g.K <= kShortK, "The short-K column kernel needs K <= 16.");
         const int64_t m_tiles = ceil_div(g.M, kShortRows), n_tiles = ceil_div(g.N, 256);
         const int64_t blocks = g.batch * m_tiles * n_tiles;
         JH_REQUIRE(blocks <= 0x7fffffff, "Too many blocks.");
         hipLaunchKernelGGL(cgemm_shortk_cols_kernel<R>, dim3((unsigned)blocks), dim3(256), 0,
                            stream, g, m_tiles, n_tiles);
     }
-    else if (g.K <= kShortK && g.M >= 1024 && g.N < 64) {
+    else if (g.variant == 3 || (g.variant == 0 && g.K <= kShortK && g.M >= 1024 && g.N < 64)) {
+        JH_REQUIRE(g.K <= kShortK, "The short-K row kernel needs K <= 16.");
         const int64_t m_tiles = ceil_div(g.M, 256);
         const int64_t blocks = g.batch * m_tiles;
         JH_REQUIRE(blocks <= 0x7fffffff, "Too many blocks.");
@@ -1471,9 +1655,12 @@ void launch_gemm(int kernel, int elem_bytes, const GemmArgs &g,
     JH_REQUIRE(g.batch <= 65535, "Batch count exceeds 65535.");
     if (kernel == kGemmF32Mfma) {
         JH_REQUIRE(elem_bytes == 8, "FP32 MFMA kernel needs complex64 data.");
-        // 128x128 tiles; 128x64 or 64x128 when N or M is at most 64 (half of
-        // a 128-wide tile would be padding).
-        const int BM = (g.M <= 64 && g.N > 64) ? 64 : 128, BN = g.N <= 64 ? 64 : 128;
+        // Variant 0: 128x128 tiles; 128x64 or 64x128 when N or M is at most
+        // 64 (half of a 128-wide tile would be padding).
+        int BM = (g.M <= 64 && g.N > 64) ? 64 : 128, BN = g.N <= 64 ? 64 : 128;
+        if (g.variant == 1) BM = 128, BN = 128;
+        if (g.variant == 2) BM = 64, BN = 128;
+        if (g.variant == 3) BM = 128, BN = 64;
         JH_REQUIRE(ceil_div(g.M, BM) * ceil_div(g.N, BN) <= 0x7fffffff, "Too many tiles.");
         dim3 grid((unsigned)(ceil_div(g.M, BM) * ceil_div(g.N, BN)), 1, (unsigned)g.batch);
         prof::Scope ps(prof::kGemmF32, stream, gemm_flops(g), gemm_bytes(g, 8));
@@ -1587,10 +1774,27 @@ constexpr int kX4mWideN = 4096;
 template <class E>
 void launch_x4m(const SplitOperand &a, const SplitOperand &b, void *C, int64_t M, int64_t N,
                 int64_t ldc, int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
-                hipStream_t stream)
+                hipStream_t stream, int variant)
 {
     constexpr int BM = 128, WM = 2, WN = 4;
-    const int BN = N >= kX4mWideN ? 192 : 128;
+    if (variant == 3) {
+        // Persistent 128x128 tiles: one work-group per CU, the grid a
+        // multiple of the XCD count.  (128x192 tiles with the register
+        // epilogue spill 106 VGPRs, so the persistent form stays at 128x128.)
+        constexpr int BN = 128;
+        const int64_t total = ceil_div(M, BM) * ceil_div(N, BN) * batch;
+        int device = 0, cus = 0;
+        JH_HIP(hipGetDevice(&device));
+        JH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        int64_t grid = std::max(kXcds, cus - cus % kXcds);
+        grid = std::min<int64_t>(grid, total);
+        hipLaunchKernelGGL((cgemm_x4m_persist_kernel<E, BM, BN, WM, WN>), dim3((unsigned)grid),
+                           dim3(64 * WM * WN), 0, stream, a, b, static_cast<float2 *>(C), M, N,
+                           ldc, sC, a_bcast ? 1 : 0, b_bcast ? 1 : 0, batch);
+        JH_HIP(hipGetLastError());
+        return;
+    }
+    const int BN = variant == 1 ? 128 : variant == 2 ? 192 : N >= kX4mWideN ? 192 : 128;
     JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
     dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);
     if (BN == 192)
@@ -1607,7 +1811,7 @@ void launch_x4m(const SplitOperand &a, const SplitOperand &b, void *C, int64_t M
 void launch_gemm_f16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                         void *C, int64_t M, int64_t N, int64_t ldc,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
-                        hipStream_t stream)
+                        hipStream_t stream, int variant)
 {
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
     JH_REQUIRE(a.planes == 4 && b.planes == 4 && a.interleaved && b.interleaved && a.scale &&
@@ -1617,13 +1821,13 @@ void launch_gemm_f16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_
     const double K = double(K_true);
     prof::Scope ps(prof::kGemmF16x4m, stream, 8.0 * double(batch) * double(M) * double(N) * K,
                    8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
-    launch_x4m<_Float16>(a, b, C, M, N, ldc, sC, batch, a_bcast, b_bcast, stream);
+    launch_x4m<_Float16>(a, b, C, M, N, ldc, sC, batch, a_bcast, b_bcast, stream, variant);
 }
 
 void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                          void *C, int64_t M, int64_t N, int64_t ldc,
                          int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
-                         hipStream_t stream)
+                         hipStream_t stream, int variant)
 {
     JH_REQUIRE(a.k_p == b.k_p, "Split operands disagree on k.");
     JH_REQUIRE(a.planes == 4 && b.planes == 4 && a.interleaved && b.interleaved,
@@ -1633,7 +1837,7 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K
     const double K = double(K_true);
     prof::Scope ps(prof::kGemmBf16x4m, stream, 8.0 * double(batch) * double(M) * double(N) * K,
                    8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
-    launch_x4m<__bf16>(a, b, C, M, N, ldc, sC, batch, a_bcast, b_bcast, stream);
+    launch_x4m<__bf16>(a, b, C, M, N, ldc, sC, batch, a_bcast, b_bcast, stream, variant);
 }
 
 void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b, int64_t K_true,

@@ -21,6 +21,7 @@ struct GemmArgs {
     int64_t M, N, K, batch;
     int64_t lda, ldb, ldc; // row strides in elements
     int64_t sA, sB, sC;    // batch strides in elements (0 = broadcast)
+    int variant = 0;       // tile / thread-mapping variant (GemmChoice)
 };
 
 // Kernel families (also reported by jet_hip_plan_info).
@@ -36,6 +37,29 @@ enum GemmKernel {
     kGemmF16x4m = 7,    // 4M complex, split-FP16 with per-row power-of-two
                         // scales, v_mfma_f32_16x16x32_f16
 };
+
+// A GEMM kernel family and its tile (or thread-mapping) variant.  Variant 0
+// lets the launcher pick by shape; the others force one form:
+//   kGemmF32Mfma              1: 128x128  2: 64x128  3: 128x64 block tiles
+//   kGemmF64Mfma              1: 128x64 block tiles
+//   kGemmBf16x4m, kGemmF16x4m 1: 128x128  2: 128x192 block tiles
+//                             3: 128x128 persistent (one work-group per
+//                             CU walking the tiles)
+//   kGemmSmall                1: one thread per output  2: short-K column
+//                             blocks (K <= 16)  3: short-K row threads (K <= 16)
+//   kGemmSmallRed, kGemmSplitK, kGemmBf16x3   1: their one form
+struct GemmChoice {
+    int kernel = kGemmSmall;
+    int variant = 0;
+};
+
+// Variants 1..gemm_variants(kernel) exist for each kernel family.
+int gemm_variants(int kernel);
+// Whether the choice computes this shape in precision mode `mode` (0 fp32,
+// 1 split-bf16 3M, 2 fp64, 3 split-bf16 4M, 4 split-fp16 4M): the data type
+// matches, the kernel's grid fits and its operand limits hold.
+bool gemm_choice_valid(int mode, int64_t M, int64_t N, int64_t K, int64_t batch,
+                       GemmChoice c);
 
 // Split-K: k range per block, and the FP64 partial-sum buffer it needs.
 constexpr int64_t kSplitKChunk = 8192;
@@ -97,9 +121,11 @@ inline int64_t split_plane_elems(int64_t batch, int64_t rows, int64_t k)
     return batch * rp * kp;
 }
 
-// Which kernel the plan uses for a given shape and precision mode
-// (0 = fp32/complex64, 1 = split-bf16 3M/complex64, 2 = fp64/complex128,
-//  3 = split-bf16 4M/complex64, 4 = split-fp16 4M/complex64).
+// The built-in kernel rule for a shape and precision mode (0 = fp32/
+// complex64, 1 = split-bf16 3M/complex64, 2 = fp64/complex128, 3 = split-bf16
+// 4M/complex64, 4 = split-fp16 4M/complex64), variant 0.  Plans use the
+// autotuned table (tuning.hpp) and fall back to this rule only for shapes
+// the table does not cover.
 int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
                        int64_t batch);
 
@@ -115,7 +141,7 @@ void launch_split_bf16(const void *src, int64_t rows, int64_t k, int64_t ld,
 void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                          void *C, int64_t M, int64_t N, int64_t ldc,
                          int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
-                         hipStream_t stream);
+                         hipStream_t stream, int variant = 0);
 // Row maxima then the scaled fp16 planes of a split-FP16 operand.
 void launch_split_f16(const void *src, int64_t rows, int64_t k, int64_t ld,
                       int64_t s, int64_t batch, const SplitOperand &dst,
@@ -126,7 +152,7 @@ void launch_split_f16_gather(const void *src, const SplitGather &g, const SplitO
 void launch_gemm_f16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                         void *C, int64_t M, int64_t N, int64_t ldc,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
-                        hipStream_t stream);
+                        hipStream_t stream, int variant = 0);
 void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b, int64_t K_true,
                         void *C, int64_t M, int64_t N, int64_t ldc,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,

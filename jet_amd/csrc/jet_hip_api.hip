@@ -9,6 +9,7 @@
 #include "network_plan.hpp"
 #include "permute_plan.hpp"
 #include "profiler.hpp"
+#include "tuning.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -173,7 +174,8 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
                 launch_split_f16_gather(b, d.gb, sb, s);
             else
                 launch_split_f16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
-            launch_gemm_f16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s);
+            launch_gemm_f16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s,
+                               d.gemm_variant);
         }
         else {
             launch_split_bf16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
@@ -183,7 +185,7 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
                                    false, s);
             else
                 launch_gemm_bf16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false,
-                                    false, s);
+                                    false, s, d.gemm_variant);
         }
     }
     else {
@@ -201,6 +203,7 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         g.sA = d.M * d.K;
         g.sB = d.N * d.K;
         g.sC = d.M * d.N;
+        g.variant = d.gemm_variant;
         launch_gemm(d.gemm_kernel, eb, g, s, ws + d.off_partials);
     }
     if (d.need_c) launch_permute(d.pc, eb, Ct, c, s);
@@ -548,6 +551,182 @@ int jet_hip_contract_noalloc(jet_hip_plan_t plan, const void *a, const void *b, 
         // is read within one contraction, so nothing needs clearing.
         char *ws = d.workspace > 0 ? scratch_setup(ctx->device) : nullptr;
         run_contraction(d, a, b, c, ws, pick_stream(ctx, stream));
+    });
+}
+
+int jet_hip_gemm_choice(jet_hip_dtype_t dtype, jet_hip_precision_t precision, int64_t m,
+                        int64_t n, int64_t k, int64_t batch, int *kernel, int *variant,
+                        int *source)
+{
+    return guarded([&] {
+        (void)elem_bytes_of(dtype);
+        JH_REQUIRE(m >= 1 && n >= 1 && k >= 1 && batch >= 1, "GEMM extents must be positive.");
+        const GemmChoice c = tuned_gemm_choice(mode_of(dtype, precision), m, n, k, batch, source);
+        if (kernel) *kernel = c.kernel;
+        if (variant) *variant = c.variant;
+    });
+}
+
+int jet_hip_tuning_set(int n, const int32_t *entries)
+{
+    return guarded([&] {
+        if (n < 0) {
+            tuning_reset();
+            return;
+        }
+        JH_REQUIRE(n == 0 || entries != nullptr, "Null tuning entries.");
+        std::vector<TuneEntry> v;
+        for (int i = 0; i < n; i++) {
+            const int32_t *r = entries + kTuneEntryInts * i;
+            JH_REQUIRE(r[0] >= 0 && r[0] <= 4, "Unknown precision mode in a tuning entry.");
+            JH_REQUIRE(r[5] >= 0 && r[5] <= kGemmF16x4m && r[6] >= 0 &&
+                           r[6] <= gemm_variants(r[5]),
+                       "Unknown GEMM kernel or variant in a tuning entry.");
+            v.push_back(TuneEntry{r[0], r[1], r[2], r[3], r[4], r[5], r[6]});
+        }
+        tuning_set(v);
+    });
+}
+
+int jet_hip_tuning_get(int capacity, int32_t *entries, int *n)
+{
+    return guarded([&] {
+        JH_REQUIRE(n != nullptr, "Null output pointer.");
+        const std::vector<TuneEntry> v = tuning_get();
+        *n = (int)v.size();
+        for (int i = 0; i < capacity && i < (int)v.size() && entries; i++) {
+            int32_t *r = entries + kTuneEntryInts * i;
+            const TuneEntry &e = v[i];
+            r[0] = e.mode, r[1] = e.lm, r[2] = e.ln, r[3] = e.lk, r[4] = e.lb;
+            r[5] = e.kernel, r[6] = e.variant;
+        }
+    });
+}
+
+int jet_hip_autotune_gemm(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
+                          jet_hip_precision_t precision, int64_t m, int64_t n, int64_t k,
+                          int64_t batch, int reps, int capacity, int32_t *kernels,
+                          int32_t *variants, float *ms, int *count)
+{
+    return guarded([&] {
+        set_device(ctx);
+        const int eb = elem_bytes_of(dtype);
+        const int mode = mode_of(dtype, precision);
+        JH_REQUIRE(m >= 1 && n >= 1 && k >= 1 && batch >= 1 && reps >= 1,
+                   "GEMM extents and repetitions must be positive.");
+        JH_REQUIRE(count != nullptr, "Null output pointer.");
+        hipStream_t s = ctx->stream;
+        // Every valid candidate, each a plan with no permutations.
+        std::vector<ContractionDesc> cands;
+        size_t ws_max = 0;
+        // MFMA kernels first: they set the bar that slower candidates are
+        // measured against.
+        const int kernels_all[] = {kGemmF16x4m, kGemmBf16x4m, kGemmBf16x3, kGemmF32Mfma,
+                                   kGemmF64Mfma, kGemmSmall,  kGemmSplitK, kGemmSmallRed};
+        // Forms not timed (far slower than another candidate there, and slow
+        // to time): one-output-per-block reductions (SmallRed, split-K)
+        // beyond 2^20 outputs; the one-thread-per-output VALU form beyond
+        // K = 2^16; short-K row threads with fewer than 256 rows or more
+        // than 4096 columns (each thread walks a whole row); block-tiled MFMA
+        // kernels with fewer tiles than CUs and K beyond 2^16 (each tile
+        // walks K alone); and every VALU form beyond 2^36 flops where an
+        // MFMA kernel remains.
+        const double flops = 8.0 * double(batch) * double(m) * double(n) * double(k);
+        const int64_t outputs = batch * m * n;
+        const bool few_tiles_long_k =
+            k > (1 << 16) && batch * ceil_div(m, 64) * ceil_div(n, 64) < 256;
+        const bool mfma_ok =
+            !few_tiles_long_k &&
+            (gemm_choice_valid(mode, m, n, k, batch, GemmChoice{kGemmF32Mfma, 1}) ||
+             gemm_choice_valid(mode, m, n, k, batch, GemmChoice{kGemmF64Mfma, 1}));
+        for (int kern : kernels_all)
+            for (int v = 1; v <= gemm_variants(kern); v++) {
+                const GemmChoice c{kern, v};
+                if (!gemm_choice_valid(mode, m, n, k, batch, c)) continue;
+                const bool valu = kern == kGemmSmall || kern == kGemmSmallRed ||
+                                  kern == kGemmSplitK;
+                if ((kern == kGemmSmallRed || kern == kGemmSplitK) && outputs > (1 << 20))
+                    continue;
+                if (kern == kGemmSmall && v == 1 && k > (1 << 16)) continue;
+                if (kern == kGemmSmall && v == 3 && (m < 256 || n > 4096)) continue;
+                if (!valu && few_tiles_long_k) continue;
+                if (valu && mfma_ok && flops > 68719476736.0) continue;
+                ContractionDesc d;
+                d.elem_bytes = eb;
+                d.mode = mode;
+                d.batch = batch, d.M = m, d.N = n, d.K = k;
+                d.size_a = batch * m * k, d.size_b = batch * n * k, d.size_c = batch * m * n;
+                d.gemm_kernel = kern;
+                d.gemm_variant = v;
+                d.flops = 8.0 * double(batch) * double(m) * double(n) * double(k);
+                layout_workspace(d);
+                ws_max = std::max(ws_max, d.workspace);
+                cands.push_back(d);
+            }
+        JH_REQUIRE(!cands.empty(), "No GEMM kernel candidate for this shape.");
+        const size_t ba = (size_t)eb * batch * m * k, bb = (size_t)eb * batch * n * k,
+                     bc = (size_t)eb * batch * m * n;
+        std::vector<void *> bufs(4, nullptr);
+        const size_t sizes[4] = {ba, bb, bc, ws_max};
+        // The operands go back to the device, not to the pool's cache: a
+        // sweep over many shapes would otherwise keep each size cached.
+        auto release = [&] {
+            for (void *p : bufs)
+                if (p) (void)hipFreeAsync(p, s);
+            (void)hipStreamSynchronize(s);
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess)
+                (void)hipMemPoolTrimTo(pool, 0);
+        };
+        std::vector<hipEvent_t> ev(2, nullptr);
+        try {
+            for (int i = 0; i < 4; i++)
+                if (sizes[i]) JH_HIP(hipMallocAsync(&bufs[i], sizes[i], s));
+            launch_fill(eb, bufs[0], batch * m * k, 0x5eedull, 2, s);
+            launch_fill(eb, bufs[1], batch * n * k, 0x5eedull + 1, 2, s);
+            JH_HIP(hipEventCreate(&ev[0]));
+            JH_HIP(hipEventCreate(&ev[1]));
+            int out = 0;
+            float best = -1.f;
+            for (const ContractionDesc &d : cands) {
+                char *ws = static_cast<char *>(bufs[3]);
+                // The warm-up run is timed too: a candidate more than four
+                // times slower than the best so far keeps that time and
+                // skips its repetitions.
+                JH_HIP(hipEventRecord(ev[0], s));
+                run_contraction(d, bufs[0], bufs[1], bufs[2], ws, s);
+                JH_HIP(hipEventRecord(ev[1], s));
+                JH_HIP(hipEventSynchronize(ev[1]));
+                float t = 0.f;
+                JH_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
+                if (best < 0.f || t <= 4.f * best) {
+                    JH_HIP(hipEventRecord(ev[0], s));
+                    for (int r = 0; r < reps; r++)
+                        run_contraction(d, bufs[0], bufs[1], bufs[2], ws, s);
+                    JH_HIP(hipEventRecord(ev[1], s));
+                    JH_HIP(hipEventSynchronize(ev[1]));
+                    JH_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
+                    t /= float(reps);
+                    if (best < 0.f || t < best) best = t;
+                }
+                if (out < capacity) {
+                    if (kernels) kernels[out] = d.gemm_kernel;
+                    if (variants) variants[out] = d.gemm_variant;
+                    if (ms) ms[out] = t;
+                }
+                out++;
+            }
+            *count = out;
+        }
+        catch (...) {
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+            release();
+            throw;
+        }
+        for (hipEvent_t e : ev)
+            if (e) JH_HIP(hipEventDestroy(e));
+        release();
     });
 }
 

