@@ -23,3 +23,14 @@ def golden_permuter():
     import json
     with open(os.path.join(ROOT, "tests", "golden", "permuter_fixtures.json")) as f:
         return json.load(f)
+
+
+@pytest.fixture
+def builtin_rule():
+    """Plans created in the test take the library's built-in kernel rule (the
+    autotuned table cleared): for tests written to exercise one particular
+    kernel.  The compiled-in table comes back afterwards."""
+    from jet_amd import _native
+    _native.check(_native.lib().jet_hip_tuning_set(0, None))
+    yield
+    _native.check(_native.lib().jet_hip_tuning_set(-1, None))

@@ -212,8 +212,10 @@ def test_contract_random_small(ctx, prec, dtype):
     (nat.PREC_FP64, np.complex128, (2, 1, 1 << 18)),
     (nat.PREC_FP64, np.complex128, (3, 5, 3000)),
 ])
-def test_gemm_kernels(ctx, prec, dtype, shape):
-    """Matrix-shaped contractions that land on each GEMM kernel."""
+def test_gemm_kernels(ctx, builtin_rule, prec, dtype, shape):
+    """Matrix-shaped contractions that land on each GEMM kernel under the
+    built-in rule (the autotuned choices are covered by test_gpu_tuning.py
+    and by every network test)."""
     M, N, K = shape
     rng = np.random.default_rng(M * 7 + N * 3 + K)
     a = rand_c(rng, M * K, dtype)
@@ -234,7 +236,7 @@ def test_gemm_kernels(ctx, prec, dtype, shape):
         assert info["kernel"] == "f16x3_4m_mfma"
 
 
-def test_gemm_more_than_2_23_columns(ctx):
+def test_gemm_more_than_2_23_columns(ctx, builtin_rule):
     """An FP32 MFMA GEMM whose N needs more than 65535 column tiles (the
     tile index is flattened): a short, very wide product, with a ragged last
     tile; sampled columns against numpy."""
@@ -377,7 +379,7 @@ def test_reduce_compensated(ctx):
     assert abs(got[0] - want.real) < 1e-6 and abs(got[1] - want.imag) < 1e-6
 
 
-def test_batch_modes_short_k(ctx):
+def test_batch_modes_short_k(ctx, builtin_rule):
     """Batched short-K products on the VALU kernels (column and row forms)."""
     rng = np.random.default_rng(14)
     for (S, M, K, N) in [(3, 100, 4, 700), (2, 3000, 6, 9)]:
@@ -393,7 +395,7 @@ def test_batch_modes_short_k(ctx):
 
 @pytest.mark.parametrize("prec", [nat.PREC_BF16X3, nat.PREC_BF16X3_4M, nat.PREC_F16X3_4M,
                                   nat.PREC_FP32])
-def test_batch_modes_mfma(ctx, prec):
+def test_batch_modes_mfma(ctx, builtin_rule, prec):
     """Batched MFMA GEMMs (batch modes in A, B and C) at tile-sized shapes."""
     rng = np.random.default_rng(13)
     S, M, K, N = 3, 160, 256, 144
@@ -408,7 +410,7 @@ def test_batch_modes_mfma(ctx, prec):
     assert rel_err(got, want) < 1e-5, (info, rel_err(got, want))
 
 
-def test_split_f16_row_scales(ctx):
+def test_split_f16_row_scales(ctx, builtin_rule):
     """Split-FP16 scales every operand row by its own power of two: rows whose
     magnitudes differ by up to 10^24 keep FP32-level relative accuracy
     (error measured per output against |A row| * |B row| * sqrt(K)), where an
@@ -433,7 +435,7 @@ def test_split_f16_row_scales(ctx):
 @pytest.mark.parametrize("M,N,K", [(333, 150, 128), (200, 131, 130), (129, 260, 256),
                                    (150, 140, 300), (140, 300, 520), (130, 129, 2048),
                                    (131, 140, 2050)])
-def test_split_f16_one_pass_groups(ctx, M, N, K):
+def test_split_f16_one_pass_groups(ctx, builtin_rule, M, N, K):
     """The one-pass split (rows of at most 2048 padded values) packs several
     short rows into one wave (16 lanes per row at K = 128, 32 at K <= 256)
     and uses 16-byte loads when K is a multiple of 8; longer rows take the
@@ -455,7 +457,7 @@ def test_split_f16_one_pass_groups(ctx, M, N, K):
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
-def test_split_f16_gathered_operands(ctx, seed):
+def test_split_f16_gathered_operands(ctx, builtin_rule, seed):
     """Split-FP16 contractions of bond-2 tensors in scrambled mode orders:
     the operand permutations are folded into the split pass (gathered LDS
     tiles) wherever the plan allows it; results against numpy in FP64, and
@@ -490,7 +492,7 @@ def test_split_f16_gathered_operands(ctx, seed):
 @pytest.mark.parametrize("dtype,prec", [(np.complex64, nat.PREC_FP32),
                                         (np.complex64, nat.PREC_F16X3_4M),
                                         (np.complex128, nat.PREC_FP64)])
-def test_split_k_scrambled_operands(ctx, dtype, prec):
+def test_split_k_scrambled_operands(ctx, builtin_rule, dtype, prec):
     """Few-output, long-K contractions (split-K GEMM, FP64 partial sums) of
     bond-2 tensors in scrambled mode orders, with a batch mode (the last
     step of a batched amplitude network); results against numpy in FP64."""
@@ -513,7 +515,7 @@ def test_split_k_scrambled_operands(ctx, dtype, prec):
     assert rel_err(got, want) < tol
 
 
-def test_split_f16_unaligned_operands(ctx):
+def test_split_f16_unaligned_operands(ctx, builtin_rule):
     """Operands at 8-byte (not 16-byte) aligned addresses, as a caller's
     complex64 sub-array may be: the split passes fall back from 16-byte
     loads, in the gathered split (scrambled B) and in the one-pass split
