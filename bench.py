@@ -69,8 +69,10 @@ SPLIT_MODES = {"3m": ("PREC_BF16X3", "bf16x3", "gemm_bf16x3_mfma", 18 / 8),
 SPLIT_LABEL = {"3m": "split-BF16 3M", "4m": "split-BF16 4M", "f16": "split-FP16 4M"}
 
 # Kernel names in rocprofv3 summaries for the event profiler's kernel families.
-ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_x4m_wide_kernel bf16"),
-                "gemm_f16x3_4m_mfma": ("cgemm_x4m_kernel f16", "cgemm_x4m_wide_kernel f16"),
+ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_x4m_wide_kernel bf16",
+                                        "cgemm_x4m_persist_kernel bf16"),
+                "gemm_f16x3_4m_mfma": ("cgemm_x4m_kernel f16", "cgemm_x4m_wide_kernel f16",
+                                       "cgemm_x4m_persist_kernel f16"),
                 "gemm_bf16x3_mfma": ("cgemm_bf16x3_kernel",),
                 "gemm_fp32_mfma": ("cgemm_embed_kernel f32 128x128", "cgemm_embed_kernel f32 128x64",
                                    "cgemm_embed_kernel f32 64x128"),
@@ -79,17 +81,20 @@ ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_x4m_wide
 
 
 def profiled_traffic(summary: str, family: str):
-    """HBM bytes per dispatch of a kernel family (dispatch-weighted over its
-    kernels), from the traffic table of a committed rocprofv3 summary under
-    profiles/ (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel
-    calibration); None if absent."""
+    """(HBM bytes per dispatch of a kernel family, dispatch-weighted over its
+    kernels; the commit the counters were read at), from the traffic table
+    of a committed rocprofv3 summary under profiles/ (FETCH_SIZE and
+    WRITE_SIZE passes, scaled by the copy-kernel calibration, and a
+    "Measured at commit" line); (None, None) if absent."""
     path = os.path.join(ROOT, "profiles", summary)
     names = ROCPROF_NAME.get(family)
     if names is None or not os.path.exists(path):
-        return None
+        return None, None
     in_traffic = False
-    total, dispatches = 0.0, 0
+    total, dispatches, commit = 0.0, 0, None
     for line in open(path):
+        if line.startswith("Measured at commit"):
+            commit = line.split()[-1].strip("`.")
         if line.startswith("## HBM traffic"):
             in_traffic = True
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
@@ -99,8 +104,8 @@ def profiled_traffic(summary: str, family: str):
                 total += n * float(cells[4]) * 1e9
                 dispatches += n
             except ValueError:
-                return None
-    return total / dispatches if dispatches else None
+                return None, commit
+    return (total / dispatches if dispatches else None), commit
 
 
 def log(msg):
@@ -302,8 +307,10 @@ def run_c1(args):
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK["bf16"], 4),
                      "hw_16bit_mfma_frac": round(achieved * SPLIT_MODES[args.split][3] /
                                                  PEAK["bf16"], 4),
-                     "traffic": profiled_traffic("c1_split_contraction.md", kname),
-                     "traffic_source": "profiles/c1_split_contraction.md"},
+                     "traffic": profiled_traffic("c1_split_contraction.md", kname)[0],
+                     "traffic_source": ("profiles/c1_split_contraction.md: rocprofv3 --pmc "
+                                        "FETCH_SIZE / WRITE_SIZE, read at commit "
+                                        f"{profiled_traffic('c1_split_contraction.md', kname)[1]}")},
     }
     return result, (a_host, b_host, la, lb)
 
@@ -350,6 +357,11 @@ C2_PLANS = {"refined": os.path.join(ROOT, "jet_amd", "data", "grid_7x7x14_seed20
 C2_BATCH_LOG2 = {"refined": 8, "min-fill": 7}
 # rocprofv3 summaries (with HBM traffic) of the c2 run on each plan.
 C2_PROFILE = {"refined": "c2_sliced_amplitude.md", "min-fill": "c2_min_fill_path.md"}
+
+
+def dname_of(prof):
+    """The kernel family with the most device time in an event profile."""
+    return max(prof.items(), key=lambda kv: kv[1]["ms"])[0]
 
 
 def hbm_copy_ceiling(ctx, gib=2, reps=10):
@@ -557,6 +569,7 @@ def run_c2(args):
     tflops32 = exec_flops * steps32 / (ms32 * 1e-3)
     sps16 = slices16 / (ms16 * 1e-3)
     sps32 = steps32 * n_batch / (ms32 * 1e-3)
+    traffic, traffic_commit = profiled_traffic(C2_PROFILE[args.c2_plan], dname_of(prof16))
     copy_bps = hbm_copy_ceiling(ctx)
     perm = prof16.get("permute_bits")
     perm_bps = perm["bytes"] / (perm["ms"] * 1e-3) if perm and perm["ms"] > 0 else None
@@ -632,8 +645,10 @@ def run_c2(args):
                      "hw_16bit_mfma_frac": (round(achieved * SPLIT_MODES[args.split][3] / peak, 4)
                                             if dname == SPLIT_MODES[args.split][2] else None),
                      "algorithmic_bytes": round(dk["bytes"] / dk["launches"]),
-                     "traffic": profiled_traffic(C2_PROFILE[args.c2_plan], dname),
-                     "traffic_source": "profiles/" + C2_PROFILE[args.c2_plan]},
+                     "traffic": traffic,
+                     "traffic_source": (f"profiles/{C2_PROFILE[args.c2_plan]}: rocprofv3 --pmc "
+                                        f"FETCH_SIZE / WRITE_SIZE of this kernel family, read "
+                                        f"at commit {traffic_commit}")},
     }, (plan, net)
 
 

@@ -74,6 +74,10 @@ __global__ __launch_bounds__(kPermThreads) void permute_bits_pairs_kernel(
         ld0[e] = bits_lds_index(uu0, 8);
         ld1[e] = bits_lds_index(uu1, 8);
     }
+    // Non-temporal loads and stores: every element is read and written once
+    // (measured on six rank-28 permutations: 4.46 TB/s against 4.23 with
+    // cached accesses, 4.41 with only the stores non-temporal).
+    typedef float f32x4v __attribute__((ext_vector_type(4)));
     for (int64_t w = blockIdx.x; w < ntiles; w += gridDim.x) {
         int64_t bin, bout;
         bits_tile_base(p, w, &bin, &bout);
@@ -81,7 +85,8 @@ __global__ __launch_bounds__(kPermThreads) void permute_bits_pairs_kernel(
 #pragma unroll
         for (int e = 0; e < PMAX; e++) {
             if (2 * (tid + e * kPermThreads) < tsize) {
-                const float4 x = *reinterpret_cast<const float4 *>(in + bin + ioff[e]);
+                const f32x4v x = __builtin_nontemporal_load(
+                    reinterpret_cast<const f32x4v *>(in + bin + ioff[e]));
                 lds[ls0[e]] = float2{x.x, x.y};
                 lds[ls1[e]] = float2{x.z, x.w};
             }
@@ -91,7 +96,8 @@ __global__ __launch_bounds__(kPermThreads) void permute_bits_pairs_kernel(
         for (int e = 0; e < PMAX; e++) {
             if (2 * (tid + e * kPermThreads) < tsize) {
                 const float2 a = lds[ld0[e]], b = lds[ld1[e]];
-                *reinterpret_cast<float4 *>(out + bout + ooff[e]) = float4{a.x, a.y, b.x, b.y};
+                __builtin_nontemporal_store(f32x4v{a.x, a.y, b.x, b.y},
+                                            reinterpret_cast<f32x4v *>(out + bout + ooff[e]));
             }
         }
     }
