@@ -121,7 +121,7 @@ int jet_hip_device_synchronize(jet_hip_ctx_t ctx);
  * out has modes out_modes (a permutation of in_modes); extents are those of
  * `in` in in_modes order.  in and out must not overlap.  Any rank <= 64 and
  * any extents; power-of-two extents take the LDS-tiled bit-permutation
- * kernel.  Identity permutations are a device copy.                       */
+ * kernel.  Identity permutations are the library's streaming device copy. */
 int jet_hip_permute(jet_hip_ctx_t ctx, jet_hip_dtype_t dtype,
                     jet_hip_layout_t layout, int rank, const int32_t *in_modes,
                     const int64_t *extents, const int32_t *out_modes,

