@@ -151,9 +151,11 @@ bool gemm_choice_valid(int mode, int64_t M, int64_t N, int64_t K, int64_t batch,
     case kGemmBf16x3:
         return mode == 1 && batch <= 65535 && ceil_div(M, 128) * ceil_div(N, 128) <= 0x7fffffff;
     case kGemmBf16x4m:
-        return mode == 3 && batch <= 65535 && ceil_div(M, 128) * ceil_div(N, 128) <= 0x7fffffff;
+        return mode == 3 && batch <= 65535 && ceil_div(M, 128) * ceil_div(N, 128) <= 0x7fffffff &&
+               ceil_div(K, kSplitKPad) * kSplitKPad <= kX4mMaxK;
     case kGemmF16x4m:
-        return mode == 4 && batch <= 65535 && ceil_div(M, 128) * ceil_div(N, 128) <= 0x7fffffff;
+        return mode == 4 && batch <= 65535 && ceil_div(M, 128) * ceil_div(N, 128) <= 0x7fffffff &&
+               ceil_div(K, kSplitKPad) * kSplitKPad <= kX4mMaxK;
     default:
         return false;
     }
@@ -170,8 +172,9 @@ int choose_gemm_kernel(int mode, int64_t M, int64_t N, int64_t K,
         // split-FP16 kernel also beats FP32 MFMA from K = 64 (1.4-2.1x on
         // 256 x 2^23 x 64 .. 16384^2 x 64), and at K = 32 on large outputs.
         if (mode == 1 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x3;
-        if (mode == 3 && M >= 128 && N >= 128 && K >= 128) return kGemmBf16x4m;
-        if (mode == 4 && M >= 128 && N >= 128 &&
+        const bool x4m_k = K <= kX4mMaxK;
+        if (mode == 3 && M >= 128 && N >= 128 && K >= 128 && x4m_k) return kGemmBf16x4m;
+        if (mode == 4 && M >= 128 && N >= 128 && x4m_k &&
             (K >= 64 || (K >= 32 && batch * M * N >= (int64_t(1) << 24))))
             return kGemmF16x4m;
         return kGemmF32Mfma;

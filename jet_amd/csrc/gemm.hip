@@ -922,6 +922,21 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_bf16x3_kernel(
     }
 }
 
+// Byte offset, from the first row of a tile, of the 16-byte chunk that load
+// lane p (tile row p / 16, LDS slot p % 16) reads in k tile 0; rows past the
+// padded operand read its last row.  It stays below 2^32 because the host
+// keeps k_p <= kX4mMaxK, so each direct-to-LDS load is a scalar tile base
+// plus one 32-bit VGPR (64-bit per-lane addresses cost 8-16 more VGPRs and
+// a 64-bit add per load).
+__device__ __forceinline__ uint32_t x4m_chunk_offset(int p, int64_t t0, int64_t rows,
+                                                     int64_t nkt)
+{
+    const int r = p >> 4, slot = p & 15;
+    int64_t m = t0 + r;
+    if (m >= rows) m = rows - 1;
+    return uint32_t(((m - t0) * nkt * 16 + (slot ^ (r & 15))) * 16);
+}
+
 // ---------------------------------------------------------------------------
 // 4M split-BF16 complex GEMM.
 // Re C = Ar.Br - Ai.Bi and Im C = Ar.Bi + Ai.Br, each real product split into
@@ -989,7 +1004,10 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_kernel(
     __shared__ vec As0[BM * 16], As1[BM * 16];
     __shared__ vec Bs0[BN * 16], Bs1[BN * 16];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    // The wave index is uniform: in a scalar register, the LDS destinations
+    // of the direct loads are scalar too.
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave / WN, wn = wave % WN;
     const int64_t bz = blockIdx.z;
     int64_t tm, tn;
@@ -999,36 +1017,26 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_kernel(
     const E *Ab = static_cast<const E *>(a.base) + (a_bcast ? 0 : bz * a.rows_p * a.k_p * 4);
     const E *Bb = static_cast<const E *>(b.base) + (b_bcast ? 0 : bz * b.rows_p * b.k_p * 4);
     const int64_t nkt_a = a.k_p / KT, nkt_b = b.k_p / KT;
+    const char *At = reinterpret_cast<const char *>(Ab + m0 * nkt_a * 128);
+    const char *Bt = reinterpret_cast<const char *>(Bb + n0 * nkt_b * 128);
 
-    // Source of this lane's chunks (elements, k tile 0).
-    int64_t offa[A_IN], offb[B_IN];
+    uint32_t offa[A_IN], offb[B_IN];
 #pragma unroll
-    for (int i = 0; i < A_IN; i++) {
-        const int p = (wave * A_IN + i) * 64 + lane;
-        const int r = p >> 4, slot = p & 15;
-        int64_t m = m0 + r;
-        if (m >= a.rows_p) m = a.rows_p - 1;
-        offa[i] = (m * nkt_a * 16 + (slot ^ (r & 15))) * 8;
-    }
+    for (int i = 0; i < A_IN; i++)
+        offa[i] = x4m_chunk_offset((wave * A_IN + i) * 64 + lane, m0, a.rows_p, nkt_a);
 #pragma unroll
-    for (int i = 0; i < B_IN; i++) {
-        const int p = (wave * B_IN + i) * 64 + lane;
-        const int r = p >> 4, slot = p & 15;
-        int64_t n = n0 + r;
-        if (n >= b.rows_p) n = b.rows_p - 1;
-        offb[i] = (n * nkt_b * 16 + (slot ^ (r & 15))) * 8;
-    }
+    for (int i = 0; i < B_IN; i++)
+        offb[i] = x4m_chunk_offset((wave * B_IN + i) * 64 + lane, n0, b.rows_p, nkt_b);
     auto issue = [&](int kt, vec *as, vec *bs) {
+        const uint32_t kb = uint32_t(kt) * 256u;
 #pragma unroll
         for (int i = 0; i < A_IN; i++)
-            __builtin_amdgcn_global_load_lds(
-                (const void *)(Ab + offa[i] + int64_t(kt) * 128),
-                (lds_void *)(as + (wave * A_IN + i) * 64), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(At + uint64_t(offa[i] + kb)),
+                                             (lds_void *)(as + (wave * A_IN + i) * 64), 16, 0, 0);
 #pragma unroll
         for (int i = 0; i < B_IN; i++)
-            __builtin_amdgcn_global_load_lds(
-                (const void *)(Bb + offb[i] + int64_t(kt) * 128),
-                (lds_void *)(bs + (wave * B_IN + i) * 64), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(Bt + uint64_t(offb[i] + kb)),
+                                             (lds_void *)(bs + (wave * B_IN + i) * 64), 16, 0, 0);
     };
 
     f32x4 cr[FM][FN], ci[FM][FN];
@@ -1154,7 +1162,8 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_wide_kernel(
     __shared__ vec As[2][BM * 16];
     __shared__ vec Bs[2][BN * 16];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6); // uniform: scalar
     const int wm = wave / WN, wn = wave % WN;
     const int64_t bz = blockIdx.z;
     int64_t tm, tn;
@@ -1163,33 +1172,33 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_wide_kernel(
     const E *Ab = static_cast<const E *>(a.base) + (a_bcast ? 0 : bz * a.rows_p * a.k_p * 4);
     const E *Bb = static_cast<const E *>(b.base) + (b_bcast ? 0 : bz * b.rows_p * b.k_p * 4);
     const int64_t nkt_a = a.k_p / KT, nkt_b = b.k_p / KT;
+    const char *At = reinterpret_cast<const char *>(Ab + m0 * nkt_a * 128);
+    const char *Bt = reinterpret_cast<const char *>(Bb + n0 * nkt_b * 128);
 
-    // Load i < A_IN moves A chunks, the others B chunks (elements, k tile 0).
-    int64_t off[L];
+    // Load i < A_IN moves A chunks, the others B chunks.
+    uint32_t off[L];
 #pragma unroll
     for (int i = 0; i < L; i++) {
         const bool isa = i < A_IN;
         const int p = (wave * (isa ? A_IN : B_IN) + (isa ? i : i - A_IN)) * 64 + lane;
-        const int r = p >> 4, slot = p & 15;
-        const int64_t rows = isa ? a.rows_p : b.rows_p;
-        int64_t m = (isa ? m0 : n0) + r;
-        if (m >= rows) m = rows - 1;
-        off[i] = (m * (isa ? nkt_a : nkt_b) * 16 + (slot ^ (r & 15))) * 8;
+        off[i] = isa ? x4m_chunk_offset(p, m0, a.rows_p, nkt_a)
+                     : x4m_chunk_offset(p, n0, b.rows_p, nkt_b);
     }
     // Loads [lo, hi) of k tile kt into buffer s, kept in place by the
     // scheduling barriers.
     auto issue = [&](int lo, int hi, int kt, int s) {
         __builtin_amdgcn_sched_barrier(0);
+        const uint32_t kb = uint32_t(kt) * 256u;
 #pragma unroll
         for (int i = 0; i < L; i++) {
             if (i < lo || i >= hi) continue;
             if (i < A_IN)
                 __builtin_amdgcn_global_load_lds(
-                    (const void *)(Ab + off[i] + int64_t(kt) * 128),
+                    (const void *)(At + uint64_t(off[i] + kb)),
                     (lds_void *)(&As[s][(wave * A_IN + i) * 64]), 16, 0, 0);
             else
                 __builtin_amdgcn_global_load_lds(
-                    (const void *)(Bb + off[i] + int64_t(kt) * 128),
+                    (const void *)(Bt + uint64_t(off[i] + kb)),
                     (lds_void *)(&Bs[s][(wave * B_IN + i - A_IN) * 64]), 16, 0, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -1355,7 +1364,8 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_persist_kernel(
     __shared__ vec As[2][BM * 16];
     __shared__ vec Bs[2][BN * 16];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6); // uniform: scalar
     const int wm = wave / WN, wn = wave % WN;
     const int64_t tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
     const int64_t tiles = tiles_m * tiles_n, total = tiles * batch;
@@ -1366,39 +1376,39 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_persist_kernel(
 
     // The tile being loaded: coordinates, operand bases, per-lane offsets.
     int64_t m0 = 0, n0 = 0, bz = 0;
-    const E *Ab = nullptr, *Bb = nullptr;
-    int64_t off[L];
+    const char *At = nullptr, *Bt = nullptr;
+    uint32_t off[L];
     auto locate = [&](int64_t tile) {
         bz = tile / tiles;
         int64_t tm, tn;
         tile_coords(tile % tiles, tiles_m, tiles_n, tm, tn);
         m0 = tm * BM;
         n0 = tn * BN;
-        Ab = static_cast<const E *>(a.base) + (a_bcast ? 0 : bz * a.rows_p * a.k_p * 4);
-        Bb = static_cast<const E *>(b.base) + (b_bcast ? 0 : bz * b.rows_p * b.k_p * 4);
+        const E *Ab = static_cast<const E *>(a.base) + (a_bcast ? 0 : bz * a.rows_p * a.k_p * 4);
+        const E *Bb = static_cast<const E *>(b.base) + (b_bcast ? 0 : bz * b.rows_p * b.k_p * 4);
+        At = reinterpret_cast<const char *>(Ab + m0 * nkt_a * 128);
+        Bt = reinterpret_cast<const char *>(Bb + n0 * nkt_b * 128);
 #pragma unroll
         for (int i = 0; i < L; i++) {
             const bool isa = i < A_IN;
             const int p = (wave * (isa ? A_IN : B_IN) + (isa ? i : i - A_IN)) * 64 + lane;
-            const int r = p >> 4, slot = p & 15;
-            const int64_t rows = isa ? a.rows_p : b.rows_p;
-            int64_t m = (isa ? m0 : n0) + r;
-            if (m >= rows) m = rows - 1;
-            off[i] = (m * (isa ? nkt_a : nkt_b) * 16 + (slot ^ (r & 15))) * 8;
+            off[i] = isa ? x4m_chunk_offset(p, m0, a.rows_p, nkt_a)
+                         : x4m_chunk_offset(p, n0, b.rows_p, nkt_b);
         }
     };
     auto issue = [&](int lo, int hi, int kt, int s) {
         __builtin_amdgcn_sched_barrier(0);
+        const uint32_t kb = uint32_t(kt) * 256u;
 #pragma unroll
         for (int i = 0; i < L; i++) {
             if (i < lo || i >= hi) continue;
             if (i < A_IN)
                 __builtin_amdgcn_global_load_lds(
-                    (const void *)(Ab + off[i] + int64_t(kt) * 128),
+                    (const void *)(At + uint64_t(off[i] + kb)),
                     (lds_void *)(&As[s][(wave * A_IN + i) * 64]), 16, 0, 0);
             else
                 __builtin_amdgcn_global_load_lds(
-                    (const void *)(Bb + off[i] + int64_t(kt) * 128),
+                    (const void *)(Bt + uint64_t(off[i] + kb)),
                     (lds_void *)(&Bs[s][(wave * B_IN + i - A_IN) * 64]), 16, 0, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -1791,6 +1801,7 @@ void launch_x4m(const SplitOperand &a, const SplitOperand &b, void *C, int64_t M
                 hipStream_t stream, int variant)
 {
     constexpr int BM = 128, WM = 2, WN = 4;
+    JH_REQUIRE(a.k_p <= kX4mMaxK, "k exceeds the 4M split kernels' limit (2^21).");
     if (variant == 3) {
         // Persistent 128x128 tiles: one work-group per CU, the grid a
         // multiple of the XCD count.  (128x192 tiles with the register

@@ -76,6 +76,9 @@ inline int64_t splitk_partials(int64_t outputs, int64_t K)
 // Rows and k of the split-BF16 operand planes are padded to these multiples.
 constexpr int kSplitRowPad = 16;
 constexpr int kSplitKPad = 32;
+// Longest padded k of the 4M split kernels: their per-lane load offsets
+// within a tile of up to 192 operand rows stay below 2^32 bytes.
+constexpr int64_t kX4mMaxK = int64_t(1) << 21;
 
 // A split-FP16 operand read straight from an unpermuted source: the
 // permutation to [batch][rows][K] is a permutation of the linear index bits

@@ -77,6 +77,20 @@ def test_exact_nearest_and_skipped_entries(table):
     assert choice(4000, 4100, 4096)[:2] == (SMALL, 1)
 
 
+def test_split_kernels_not_chosen_past_their_k_limit(table):
+    """The 4M split kernels keep 32-bit per-lane load offsets, valid for a
+    padded k up to 2^21: beyond it neither the rule nor a table entry that
+    names them may pick them."""
+    f16 = dict(prec=nat.PREC_F16X3_4M)
+    nat.check(nat.lib().jet_hip_tuning_set(0, None))
+    assert choice(256, 256, 1 << 21, **f16)[0] == F16
+    assert choice(256, 256, (1 << 21) + 1, **f16)[0] == F32
+    assert choice(256, 256, 1 << 22, **f16)[0] == F32
+    nat.check(set_table([[4, 8, 8, 22, 0, F16, 2], [4, 8, 8, 21, 0, F16, 1]]))
+    assert choice(256, 256, 1 << 21, **f16) == (F16, 1, 1)
+    assert choice(256, 256, 1 << 22, **f16)[0] != F16
+
+
 def test_malformed_entries_refused(table):
     assert set_table([[9, 1, 1, 1, 0, F32, 1]]) != 0
     assert "precision mode" in nat.lib().jet_hip_last_error().decode()
