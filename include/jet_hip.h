@@ -179,7 +179,8 @@ int jet_hip_contract_noalloc(jet_hip_plan_t plan, const void *a,
  * 4M, 6 split-K with FP64 partials, 7 split-FP16 4M.  Variants: kernel 2
  * 1 = 128x128 / 2 = 64x128 / 3 = 128x64 block tiles; kernels 5 and 7
  * 1 = 128x128 / 2 = 128x192 block tiles / 3 = 128x128 tiles walked by a
- * persistent grid of one work-group per CU; kernel 0 1 = one thread per output
+ * persistent grid of one work-group per CU / 4 = 192x128 block tiles (long
+ * side along M); kernel 0 1 = one thread per output
  * / 2 = short-K column blocks / 3 = short-K row threads (K <= 16); every
  * other kernel 1.
  *

@@ -122,7 +122,7 @@ int gemm_variants(int kernel)
         return 3;
     case kGemmBf16x4m:
     case kGemmF16x4m:
-        return 3;
+        return 4;
     case kGemmSmall:
         return 3;
     default:

@@ -1792,7 +1792,9 @@ void launch_split_f16_gather(const void *src, const SplitGather &g, const SplitO
 
 // 4M kernels: 128x192 block tiles (cgemm_x4m_wide_kernel) where N fills
 // the last column tile to within 5% (N >= 4096), else 128x128; 8 waves (two
-// per SIMD) in a 2x4 grid either way.
+// per SIMD) in a 2x4 grid either way.  Variants: 1 128x128, 2 128x192,
+// 3 persistent 128x128, 4 192x128 (the wide kernel with its long side along
+// M, for products with long M and N of 128-256).
 constexpr int kX4mWideN = 4096;
 
 template <class E>
@@ -1819,12 +1821,22 @@ void launch_x4m(const SplitOperand &a, const SplitOperand &b, void *C, int64_t M
         JH_HIP(hipGetLastError());
         return;
     }
+    const int vec_store = reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % 2 == 0 &&
+                          (batch == 1 || sC % 2 == 0);
+    if (variant == 4) {
+        constexpr int TM = 192, TN = 128;
+        JH_REQUIRE(ceil_div(M, TM) * ceil_div(N, TN) <= 0x7fffffff, "Too many tiles.");
+        dim3 grid((unsigned)(ceil_div(M, TM) * ceil_div(N, TN)), 1, (unsigned)batch);
+        hipLaunchKernelGGL((cgemm_x4m_wide_kernel<E, TM, TN, WM, WN>), grid, dim3(64 * WM * WN),
+                           0, stream, a, b, static_cast<float2 *>(C), M, N, ldc, sC,
+                           a_bcast ? 1 : 0, b_bcast ? 1 : 0, vec_store);
+        JH_HIP(hipGetLastError());
+        return;
+    }
     const int BN = variant == 1 ? 128 : variant == 2 ? 192 : N >= kX4mWideN ? 192 : 128;
     JH_REQUIRE(ceil_div(M, BM) * ceil_div(N, BN) <= 0x7fffffff, "Too many tiles.");
     dim3 grid((unsigned)(ceil_div(M, BM) * ceil_div(N, BN)), 1, (unsigned)batch);
     if (BN == 192) {
-        const int vec_store = reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % 2 == 0 &&
-                              (batch == 1 || sC % 2 == 0);
         hipLaunchKernelGGL((cgemm_x4m_wide_kernel<E, BM, 192, WM, WN>), grid, dim3(64 * WM * WN),
                            0, stream, a, b, static_cast<float2 *>(C), M, N, ldc, sC,
                            a_bcast ? 1 : 0, b_bcast ? 1 : 0, vec_store);

@@ -138,3 +138,27 @@ def test_persistent_split_walks_many_tiles(ctx, mode, restore_table):
         got, info = contract(ctx, a, ma, sa, b, mb, sb, mc, precision=prec)
         assert info["kernel"] == ("f16x3_4m_mfma" if mode == "f16x3" else "bf16x3_4m_mfma")
         _check_batched(got, a, sa, b, sb, ma, mb, mc, tol, (mode, M, N, K, B))
+
+
+@pytest.mark.parametrize("mode", ["f16x3", "bf16x3_4m"])
+def test_tall_split_tiles(ctx, mode, restore_table):
+    """The 192x128 split tile (variant 4, long side along M) over many tiles
+    with ragged M and N edges, and with a batch mode, against numpy in
+    complex128."""
+    dtype, prec, mode_id, tol = MODES[mode]
+    rng = np.random.default_rng(23)
+    # 5000 x 300 x 96 (27 x 3 tiles, both edges partial) and batch 4 of
+    # 1000 x 130 x 64
+    cases = [([0, 1], [5000, 96], [2, 1], [300, 96], [0, 2]),
+             ([3, 0, 1], [4, 1000, 64], [3, 2, 1], [4, 130, 64], [3, 0, 2])]
+    for ma, sa, mb, sb, mc in cases:
+        M, N, K, B = _shape(ma, sa, mb, sb, mc)
+        lm, ln, lk, lbt = (int(round(np.log2(x))) for x in (M, N, K, B))
+        row = (ctypes.c_int32 * 7)(mode_id, lm, ln, lk, lbt, 7 if mode == "f16x3" else 5, 4)
+        nat.check(nat.lib().jet_hip_tuning_set(1, row))
+        a = rand_c(rng, int(np.prod(sa)), dtype)
+        b = rand_c(rng, int(np.prod(sb)), dtype)
+        say(f"tall split tile {mode}: {M}x{N}x{K} batch {B}")
+        got, info = contract(ctx, a, ma, sa, b, mb, sb, mc, precision=prec)
+        assert info["kernel"] == ("f16x3_4m_mfma" if mode == "f16x3" else "bf16x3_4m_mfma")
+        _check_batched(got, a, sa, b, sb, ma, mb, mc, tol, (mode, M, N, K, B))
