@@ -600,6 +600,83 @@ __global__ __launch_bounds__(256) void split_f16_rows_kernel(
     }
 }
 
+// Rows of kSplitOnePassK < k_p <= kSplitBlockRowK padded values: one
+// work-group per row holds it in registers (CH 8-value chunks per thread,
+// chunk c * 256 + thread, so each load instruction of a wave reads 4 KiB of
+// the row in order), reduces its maximum across the work-group and writes
+// the planes and the scale: one read of the operand instead of the two-pass
+// path's two.  `vec` as in split_f16_rows_kernel.
+constexpr int kSplitBlockRowK = 16384;
+
+template <int CH>
+__global__ __launch_bounds__(256) void split_f16_block_rows_kernel(
+    const float2 *__restrict__ src, int64_t rows, int64_t K, int64_t ld, int64_t s,
+    SplitOperand dst, int vec)
+{
+    __shared__ uint32_t red[4];
+    const int tid = threadIdx.x;
+    const int64_t row = blockIdx.x;
+    const int64_t b = row / dst.rows_p, r = row % dst.rows_p;
+    const bool in_row = r < rows;
+    const int kq_n = int(dst.k_p / 8);
+    const float2 *rp = src + b * s + r * ld;
+    float2 x[CH][8];
+    uint32_t m = 0u;
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+        const int kq = c * 256 + tid;
+        const bool any = in_row && kq < kq_n && int64_t(kq) * 8 < K;
+        if (vec && any) {
+            const float4 *q = reinterpret_cast<const float4 *>(rp + int64_t(kq) * 8);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const float4 v = q[e];
+                x[c][2 * e] = float2{v.x, v.y};
+                x[c][2 * e + 1] = float2{v.z, v.w};
+            }
+        }
+        else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int64_t k = int64_t(kq) * 8 + e;
+                x[c][e] = (any && k < K) ? rp[k] : float2{0.f, 0.f};
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+            m = max(m, max(__float_as_uint(x[c][e].x) & 0x7fffffffu,
+                           __float_as_uint(x[c][e].y) & 0x7fffffffu));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+    if ((tid & 63) == 0) red[tid >> 6] = m;
+    __syncthreads();
+    m = max(max(red[0], red[1]), max(red[2], red[3]));
+    const int sh = in_row ? row_shift(m) : 0;
+    const float sc = __builtin_ldexpf(1.f, sh);
+    if (tid == 0) dst.scale[row] = __builtin_ldexpf(1.f, -sh);
+    _Float16 *base = static_cast<_Float16 *>(dst.base);
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+        const int kq = c * 256 + tid;
+        if (kq >= kq_n) continue;
+        f16x8 o[4];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            _Float16 h, l;
+            split2h(x[c][e].x * sc, h, l);
+            o[0][e] = h;
+            o[1][e] = l;
+            split2h(x[c][e].y * sc, h, l);
+            o[2][e] = h;
+            o[3][e] = l;
+        }
+        const int64_t off = (row * (dst.k_p / 32) + kq / 4) * 128 + (kq % 4) * 8;
+#pragma unroll
+        for (int p = 0; p < 4; p++) *reinterpret_cast<f16x8 *>(base + off + p * 32) = o[p];
+    }
+}
+
 // Split-FP16 planes gathered from an unpermuted source (SplitGather): one
 // work-group per tile of 2^tb elements (all K values of 2^xr rows).
 //   1. the tile is read in source order, two neighbouring elements per
@@ -1752,6 +1829,22 @@ void launch_split_f16(const void *src, int64_t rows, int64_t k, int64_t ld,
         hipLaunchKernelGGL(split_f16_rows_kernel, dim3((unsigned)ceil_div(nrow, rows_per_block)),
                            dim3(256), 0, stream, static_cast<const float2 *>(src), rows, k, ld, s,
                            dst, lg, vec);
+        JH_HIP(hipGetLastError());
+        return;
+    }
+    if (dst.k_p <= kSplitBlockRowK) {
+        prof::Scope ps(prof::kSplitF16, stream, 0.0,
+                       8.0 * double(batch) * double(rows) * double(k) +
+                           8.0 * double(nrow) * double(dst.k_p));
+        const int vec = (reinterpret_cast<uintptr_t>(src) % 16 == 0 && ld % 2 == 0 &&
+                         s % 2 == 0 && k % 8 == 0) ? 1 : 0;
+        JH_REQUIRE(nrow <= 0x7fffffff, "Too many rows.");
+        if (dst.k_p <= 8192)
+            hipLaunchKernelGGL(split_f16_block_rows_kernel<4>, dim3((unsigned)nrow), dim3(256), 0,
+                               stream, static_cast<const float2 *>(src), rows, k, ld, s, dst, vec);
+        else
+            hipLaunchKernelGGL(split_f16_block_rows_kernel<8>, dim3((unsigned)nrow), dim3(256), 0,
+                               stream, static_cast<const float2 *>(src), rows, k, ld, s, dst, vec);
         JH_HIP(hipGetLastError());
         return;
     }
