@@ -1557,9 +1557,29 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_persist_kernel(
             __builtin_amdgcn_s_waitcnt(kWaitLoads);
             __syncthreads();
         }
-        // Every wave is past its last LDS read of this tile: start the next
-        // tile's first k step, then store this one.
+        // Every wave is past its last LDS read of this tile.  The row and
+        // column scales are read first (their loads then complete before the
+        // next tile's are issued, so waiting for them does not wait for
+        // those), then the next tile's first k step is issued, then this
+        // tile is stored; full tiles store without bounds checks.
         const int64_t cm0 = m0, cn0 = n0, cbz = bz;
+        const bool full = cm0 + BM <= M && cn0 + BN <= N;
+        const float *sa = a.scale ? a.scale + (a_bcast ? 0 : cbz) * a.rows_p : nullptr;
+        const float *sb = b.scale ? b.scale + (b_bcast ? 0 : cbz) * b.rows_p : nullptr;
+        float fa[FM][4], fb[FN];
+#pragma unroll
+        for (int fm = 0; fm < FM; fm++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = cm0 + wm * WTM + fm * 16 + 4 * q + r;
+                fa[fm][r] = (sa && row < M) ? sa[row] : 1.f;
+            }
+#pragma unroll
+        for (int fn = 0; fn < FN; fn++) {
+            const int64_t col = cn0 + wn * WTN + fn * 16 + j;
+            fb[fn] = (sb && col < N) ? sb[col] : 1.f;
+        }
+        __builtin_amdgcn_s_waitcnt(kWaitLoads);
         const int64_t next = t + gridDim.x;
         const bool more = next < total;
         if (more) {
@@ -1567,27 +1587,21 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_x4m_persist_kernel(
             issue(0, L, 0, 0);
         }
         float2 *C = Cout + cbz * sC;
-        const float *sa = a.scale ? a.scale + (a_bcast ? 0 : cbz) * a.rows_p : nullptr;
-        const float *sb = b.scale ? b.scale + (b_bcast ? 0 : cbz) * b.rows_p : nullptr;
+        // Power-of-two factors, applied one at a time so that their product
+        // cannot leave the float range.
 #pragma unroll
-        for (int fm = 0; fm < FM; fm++) {
+        for (int fm = 0; fm < FM; fm++)
 #pragma unroll
             for (int fn = 0; fn < FN; fn++) {
                 const int64_t col = cn0 + wn * WTN + fn * 16 + j;
-                const float fb = (sb && col < N) ? sb[col] : 1.f;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int64_t row = cm0 + wm * WTM + fm * 16 + 4 * q + r;
-                    if (row < M && col < N) {
-                        // Power-of-two factors, applied one at a time so
-                        // that their product cannot leave the float range.
-                        const float fa = sa ? sa[row] : 1.f;
-                        C[row * ldc + col] =
-                            float2{cr[fm][fn][r] * fa * fb, ci[fm][fn][r] * fa * fb};
-                    }
+                    if (full || (row < M && col < N))
+                        C[row * ldc + col] = float2{cr[fm][fn][r] * fa[fm][r] * fb[fn],
+                                                    ci[fm][fn][r] * fa[fm][r] * fb[fn]};
                 }
             }
-        }
         if (!more) break;
         t = next;
     }
