@@ -69,9 +69,11 @@ SPLIT_MODES = {"3m": ("PREC_BF16X3", "bf16x3", "gemm_bf16x3_mfma", 18 / 8),
 SPLIT_LABEL = {"3m": "split-BF16 3M", "4m": "split-BF16 4M", "f16": "split-FP16 4M"}
 
 # Kernel names in rocprofv3 summaries for the event profiler's kernel families.
-ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_x4m_wide_kernel bf16",
+ROCPROF_NAME = {"gemm_bf16x3_4m_mfma": ("cgemm_x4m_kernel bf16", "cgemm_x4m_wide_kernel bf16 128x192",
+                                        "cgemm_x4m_wide_kernel bf16 192x128",
                                         "cgemm_x4m_persist_kernel bf16"),
-                "gemm_f16x3_4m_mfma": ("cgemm_x4m_kernel f16", "cgemm_x4m_wide_kernel f16",
+                "gemm_f16x3_4m_mfma": ("cgemm_x4m_kernel f16", "cgemm_x4m_wide_kernel f16 128x192",
+                                       "cgemm_x4m_wide_kernel f16 192x128",
                                        "cgemm_x4m_persist_kernel f16"),
                 "gemm_bf16x3_mfma": ("cgemm_bf16x3_kernel",),
                 "gemm_fp32_mfma": ("cgemm_embed_kernel f32 128x128", "cgemm_embed_kernel f32 128x64",
