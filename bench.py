@@ -852,14 +852,13 @@ def run_c3(args):
     jobs = []
     for (r, c, d), n_amp, seed in C3_CIRCUITS:
         circ = circuits.random_grid_circuit(r, c, d, seed=seed)
-        net = circuits.amplitude_network(circ)
-        dims = {l: 2 for ix in net.indices for l in ix}
-        path = paths.best_minfill_path(net.indices, dims, trials=8, seed=1)
+        # min-fill path after subtree reconfiguration (jet_amd/data/amplitude_paths.json)
+        path = paths.load_or_make_amplitude_path(r, c, d, seed)
         bits = [tuple(int(x) for x in rng.integers(0, 2, r * c)) for _ in range(n_amp)]
         jobs.append(AmplitudeJob(circ, bits, path))
-    plan, circ2, net2 = c2_problem("min-fill")
+    plan, circ2, net2 = c2_problem(args.c2_plan)
     sliced = plan["sliced"]
-    nb = C2_BATCH_LOG2["min-fill"] if args.batch_log2 is None else args.batch_log2
+    nb = C2_BATCH_LOG2[args.c2_plan] if args.batch_log2 is None else args.batch_log2
     batch = sliced[len(sliced) - nb:] if nb else []
     per_exec = 2 ** len(batch)
     n_exec = max(1, args.c3_slices // per_exec)
@@ -892,6 +891,14 @@ def run_c3(args):
     forms, tuning = ub.forms(), ub.tuning
     ub.close()
     del ub
+    # The same batch on FP32 MFMA: the complex64 error floor the split run is
+    # judged beside.
+    ub32 = UnevenBatch(jobs, dtype=np.complex64, precision="fp32", ctx=ctx)
+    ub32.launch()
+    ctx.synchronize()
+    amps32 = [ub32.amplitudes(j) for j in range(len(jobs) - 1)]
+    ub32.close()
+    del ub32
 
     # FP64 reference: every amplitude, and the first 16 c2 slices (slice order
     # does not depend on how many sliced indices are batched).
@@ -901,15 +908,19 @@ def run_c3(args):
     ub64 = UnevenBatch(ref_jobs, dtype=np.complex128, precision="fp64", ctx=ctx)
     ub64.launch()
     ctx.synchronize()
-    errs = []
+    errs, errs32, norm_errs = [], [], []
     for j, a in enumerate(amps):
         ref = ub64.amplitudes(j)
         errs.extend((np.abs(a - ref) / np.abs(ref)).tolist())
+        errs32.extend((np.abs(amps32[j] - ref) / np.abs(ref)).tolist())
+        norm_errs.append(float(np.linalg.norm(a - ref) / np.linalg.norm(ref)))
     ref_s = ub64.slice_results(len(jobs) - 1)
     slice_err = float(np.linalg.norm(slices[:len(ref_s)] - ref_s) / np.linalg.norm(ref_s))
     ub64.close()
-    errs = np.array(errs)
-    log(f"c3: per-amplitude rel err max {errs.max():.2e} median {np.median(errs):.2e}; "
+    errs, errs32 = np.array(errs), np.array(errs32)
+    log(f"c3: per-amplitude rel err max {errs.max():.2e} median {np.median(errs):.2e} "
+        f"(fp32 MFMA: max {errs32.max():.2e} median {np.median(errs32):.2e}); "
+        f"per circuit normwise {', '.join(f'{x:.2e}' for x in norm_errs)}; "
         f"c2 slices {slice_err:.2e}")
     return {
         "metric": METRIC,
@@ -924,7 +935,10 @@ def run_c3(args):
                    "amplitudes": n_amp, "c2_slices": n_exec * per_exec},
         "forms": forms,
         "tuning_ms": tuning,
-        "rel_err_vs_fp64": {"amplitude_max": float(errs.max()),
+        "rel_err_vs_fp64": {"per_circuit_normwise": norm_errs,
+                            "fp32_amplitude_max": float(errs32.max()),
+                            "fp32_amplitude_median": float(np.median(errs32)),
+                            "amplitude_max": float(errs.max()),
                             "amplitude_median": float(np.median(errs)),
                             "per_amplitude": [float(x) for x in errs],
                             "c2_first_16_slices": slice_err},
@@ -945,9 +959,8 @@ def run_c4(args):
     from jet_amd.gradient import AmplitudeLoss, ParameterizedAmplitude
 
     circ = circuits.random_grid_circuit(6, 6, 10, seed=2026, random_angles=True)
-    net = circuits.amplitude_network(circ)
-    dims = {l: 2 for ix in net.indices for l in ix}
-    path = paths.best_minfill_path(net.indices, dims, trials=8, seed=1)
+    # min-fill path after subtree reconfiguration (jet_amd/data/amplitude_paths.json)
+    path = paths.load_or_make_amplitude_path(6, 6, 10, 2026, random_angles=True)
 
     def timed(prob):
         theta = torch.tensor(prob.theta, dtype=torch.float64, requires_grad=True)

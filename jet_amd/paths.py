@@ -766,6 +766,84 @@ def refine_batched_path(indices, dims, path: Path, sliced: Sequence[str], batch_
     return path
 
 
+def amplitude_step_seconds(log2_m: float, log2_n: float, log2_k: float) -> float:
+    """mi355x_step_seconds of one unbatched contraction (its launch cost
+    not shared with other slices)."""
+    return mi355x_step_seconds(log2_m, log2_n, log2_k, batch_bits=0.0)
+
+
+def reconfigured_minfill_path(indices, dims, trials: int = 8, seed: int = 0,
+                              subtree_sizes: Sequence[int] = (8, 10), passes: int = 6,
+                              step_cost=amplitude_step_seconds) -> Path:
+    """best_minfill_path, then subtree reconfiguration (reconfigure_path) at
+    each subtree size in turn, first against PathInfo flops, then against
+    step_cost (the time model of these kernels; None skips that stage), the
+    largest intermediate never above the min-fill path's.  The flops stage
+    alone removes most of the min-fill path's flops on grid-circuit
+    amplitudes (6x6 depth 12: 4.1e12 to 3.1e11) but leaves memory-bound
+    steps on 2^30-element intermediates; the time stage trades some of
+    those flops back for smaller, MFMA-friendly steps (modelled 69 -> 44 ->
+    19 ms, largest intermediate 2^28)."""
+    path = best_minfill_path(indices, dims, trials=trials, seed=seed)
+    limit = path_stats(indices, dims, path).max_log2_size
+    for cost in (None, step_cost) if step_cost is not None else (None,):
+        for size in subtree_sizes:
+            path = reconfigure_path(indices, dims, path, subtree_size=size, passes=passes,
+                                    max_log2_size=limit, step_cost=cost)
+    return path
+
+
+def modelled_amplitude_seconds(indices, dims, path: Path) -> float:
+    """Modelled seconds of one amplitude contraction along `path`
+    (amplitude_step_seconds summed over its steps)."""
+    return _Tree(indices, dims, path, [], 0.0, amplitude_step_seconds).total()
+
+
+AMPLITUDE_PATHS = "amplitude_paths.json"
+
+
+def _amplitude_key(rows, cols, depth, seed, random_angles):
+    return f"{rows}x{cols}x{depth}_seed{seed}" + ("_angles" if random_angles else "")
+
+
+def amplitude_path_entry(rows, cols, depth, seed, random_angles=False, trials=8):
+    """{"path", "pathinfo_flops", "max_log2_size", "modelled_s" and the same
+    of the min-fill path it starts from} of one amplitude of a seeded grid
+    circuit (jet_amd.circuits) along reconfigured_minfill_path."""
+    from . import circuits
+    circ = circuits.random_grid_circuit(rows, cols, depth, seed=seed,
+                                        random_angles=random_angles)
+    net = circuits.amplitude_network(circ)
+    dims = {l: 2 for ix in net.indices for l in ix}
+    base = best_minfill_path(net.indices, dims, trials=trials, seed=1)
+    path = reconfigured_minfill_path(net.indices, dims, trials=trials, seed=1)
+    st, sb = path_stats(net.indices, dims, path), path_stats(net.indices, dims, base)
+    return {"path": [list(p) for p in path], "pathinfo_flops": st.total_flops,
+            "max_log2_size": st.max_log2_size,
+            "modelled_s": modelled_amplitude_seconds(net.indices, dims, path),
+            "minfill_pathinfo_flops": sb.total_flops, "minfill_max_log2_size": sb.max_log2_size,
+            "minfill_modelled_s": modelled_amplitude_seconds(net.indices, dims, base)}
+
+
+def load_or_make_amplitude_path(rows, cols, depth, seed, random_angles=False,
+                                directory: Optional[str] = None) -> Path:
+    """The path of one amplitude of a seeded grid circuit from
+    jet_amd/data/amplitude_paths.json (written by `python -m jet_amd.paths
+    --amplitude-paths`), else computed here (amplitude_path_entry)."""
+    import json
+    import os
+    directory = directory or os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+    name = os.path.join(directory, AMPLITUDE_PATHS)
+    key = _amplitude_key(rows, cols, depth, seed, random_angles)
+    if os.path.exists(name):
+        with open(name) as f:
+            table = json.load(f)
+        if key in table:
+            return [tuple(p) for p in table[key]["path"]]
+    return [tuple(p) for p in
+            amplitude_path_entry(rows, cols, depth, seed, random_angles)["path"]]
+
+
 def _sliced_trial(args):
     indices, dims, t, seed, n_slices, max_log2_size = args
     p = minfill_trial(indices, dims, t, seed)
@@ -855,7 +933,26 @@ if __name__ == "__main__":
                     help="--refine: growth allowed for the largest batched intermediate")
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--out", help="where --refine writes (default: over its input)")
+    ap.add_argument("--amplitude-paths", action="store_true",
+                    help="write jet_amd/data/amplitude_paths.json: reconfigured min-fill "
+                         "paths of the uneven-batch circuits (5x5 depth 14 seed 51, 6x6 "
+                         "depth 12 seed 52, 6x7 depth 10 seed 53) and of the gradient "
+                         "circuit (6x6 depth 10 seed 2026, random angles)")
     a = ap.parse_args()
+    if a.amplitude_paths:
+        d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+        table = {}
+        for (r, c, dp, sd, ra) in [(5, 5, 14, 51, False), (6, 6, 12, 52, False),
+                                    (6, 7, 10, 53, False), (6, 6, 10, 2026, True)]:
+            e = amplitude_path_entry(r, c, dp, sd, ra)
+            table[_amplitude_key(r, c, dp, sd, ra)] = e
+            print(_amplitude_key(r, c, dp, sd, ra),
+                  "pathinfo flops %.3e (min-fill %.3e) max %d, modelled %.2f ms (min-fill %.2f)" % (
+                      e["pathinfo_flops"], e["minfill_pathinfo_flops"], e["max_log2_size"],
+                      e["modelled_s"] * 1e3, e["minfill_modelled_s"] * 1e3), flush=True)
+        with open(os.path.join(d, AMPLITUDE_PATHS), "w") as f:
+            json.dump(table, f)
+        raise SystemExit(0)
     if a.refine:
         from . import circuits
         with open(a.refine) as f:

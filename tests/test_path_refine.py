@@ -178,3 +178,59 @@ def test_refine_batched_path_small_circuit():
     assert t1.total() <= t0.total() * (1 + 1e-12)
     assert t1.max_log2size() <= t0.max_log2size() + 1.0
     assert t1.mask[t1.root] == t1.open        # only the batched indices remain open
+
+
+def test_reconfigured_minfill_path_amplitude():
+    """reconfigured_minfill_path on a 4x4 depth-8 amplitude: a valid SSA
+    path, no larger intermediate than the min-fill path it starts from and
+    no more PathInfo flops after the flops stage alone, and the oracle
+    contracted along it gives the statevector amplitude."""
+    import oracle
+    circ = circuits.random_grid_circuit(4, 4, 8, seed=9)
+    net = circuits.amplitude_network(circ)
+    dims = {l: 2 for ix in net.indices for l in ix}
+    base = paths.best_minfill_path(net.indices, dims, trials=4, seed=1)
+    flops_only = paths.reconfigured_minfill_path(net.indices, dims, trials=4, seed=1,
+                                                 subtree_sizes=(6, 8), passes=3, step_cost=None)
+    b = paths.path_stats(net.indices, dims, base)
+    assert paths.path_stats(net.indices, dims, flops_only).total_flops <= b.total_flops
+    path = paths.reconfigured_minfill_path(net.indices, dims, trials=4, seed=1,
+                                           subtree_sizes=(6, 8), passes=3)
+    _valid_ssa(len(net.indices), path)
+    assert paths.path_stats(net.indices, dims, path).max_log2_size <= b.max_log2_size
+    tensors = list(zip(net.indices, net.shapes, [d.astype(np.complex128) for d in net.data]))
+    _, _, amp = oracle.contract_path(tensors, path)
+    want = circuits.amplitude_statevector(circ)
+    assert abs(complex(np.asarray(amp).reshape(-1)[0]) - want) < 1e-12 * max(1.0, abs(want))
+
+
+def test_committed_amplitude_paths_are_consistent():
+    """jet_amd/data/amplitude_paths.json (the uneven-batch and gradient
+    circuits): each path is a valid SSA path of its seeded amplitude
+    network, its recorded PathInfo flops and largest intermediate match
+    path_stats, and its modelled time is below the min-fill path's."""
+    import json
+    import os
+    name = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "jet_amd", "data", paths.AMPLITUDE_PATHS)
+    with open(name) as f:
+        table = json.load(f)
+    assert set(table) == {"5x5x14_seed51", "6x6x12_seed52", "6x7x10_seed53",
+                          "6x6x10_seed2026_angles"}
+    for key, e in table.items():
+        dims_txt, seed_txt = key.split("_seed")
+        r, c, d = (int(x) for x in dims_txt.split("x"))
+        seed = int(seed_txt.split("_")[0])
+        circ = circuits.random_grid_circuit(r, c, d, seed=seed,
+                                            random_angles=key.endswith("_angles"))
+        net = circuits.amplitude_network(circ)
+        dims = {l: 2 for ix in net.indices for l in ix}
+        path = [tuple(p) for p in e["path"]]
+        _valid_ssa(len(net.indices), path)
+        st = paths.path_stats(net.indices, dims, path)
+        assert st.total_flops == e["pathinfo_flops"] and st.max_log2_size == e["max_log2_size"]
+        assert e["modelled_s"] == pytest.approx(paths.modelled_amplitude_seconds(
+            net.indices, dims, path), rel=1e-12)
+        assert e["modelled_s"] < e["minfill_modelled_s"]
+        assert e["max_log2_size"] <= e["minfill_max_log2_size"]
+        assert paths.load_or_make_amplitude_path(r, c, d, seed, key.endswith("_angles")) == path
