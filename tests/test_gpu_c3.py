@@ -73,3 +73,33 @@ def test_uneven_batch_matches_cpu_fixtures(ctx, prec, dtype, tol):
     ub.close()
     say(f"  worst amplitude error {worst:.2e}, c2 slices {err_s:.2e}")
     assert err_s < tol
+
+
+def test_uneven_batch_reconfigured_paths(ctx):
+    """The amplitude jobs along the committed reconfigured paths
+    (jet_amd/data/amplitude_paths.json, the paths the benchmark uses) in
+    split-FP16, against the same CPU fixtures: normwise error per circuit
+    below 1e-5, and every amplitude within 2e-5 relative (the largest
+    relative errors fall on the smallest amplitudes: FP32 MFMA on the same
+    batch reaches 1.2e-5 there)."""
+    with open(os.path.join(GOLDEN, "c3_amplitudes_fp64.json")) as f:
+        fx = json.load(f)
+    jobs, want = [], []
+    for c in fx["circuits"]:
+        circ = circuits.random_grid_circuit(c["rows"], c["cols"], c["depth"], seed=c["seed"])
+        path = paths.load_or_make_amplitude_path(c["rows"], c["cols"], c["depth"], c["seed"])
+        bits = [tuple(int(x) for x in b) for b in c["bitstrings"]]
+        jobs.append(AmplitudeJob(circ, bits, path))
+        want.append(np.array([complex(*v) for v in c["amplitudes"]]))
+    say(f"c3 amplitudes along reconfigured paths: {sum(len(w) for w in want)} amplitudes")
+    ub = UnevenBatch(jobs, dtype=np.complex64, precision="f16x3", ctx=ctx)
+    ub.launch()
+    ctx.synchronize()
+    for j, w in enumerate(want):
+        got = ub.amplitudes(j).astype(np.complex128)
+        norm = np.linalg.norm(got - w) / np.linalg.norm(w)
+        rel = np.abs(got - w) / np.abs(w)
+        say(f"  circuit {j}: normwise {norm:.2e}, worst amplitude {rel.max():.2e}")
+        assert norm < 1e-5, (j, norm)
+        assert rel.max() < 2e-5, (j, rel.max())
+    ub.close()
