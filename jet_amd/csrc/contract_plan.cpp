@@ -115,6 +115,74 @@ bool plan_split_gather(const std::vector<int32_t> &src, const std::vector<int64_
     return j == g.tb;
 }
 
+bool plan_fused_small(const std::vector<int32_t> &a, const std::vector<int64_t> &ea,
+                      const std::vector<int32_t> &b, const std::vector<int64_t> &eb,
+                      const std::vector<int32_t> &c, const std::vector<int32_t> &batch,
+                      const std::vector<int32_t> &left, const std::vector<int32_t> &right,
+                      const std::vector<int32_t> &common, FusedSmall &f)
+{
+    auto lg = [](int64_t e) {
+        int l = 0;
+        while ((int64_t(1) << l) < e) l++;
+        return (int64_t(1) << l) == e ? l : -1;
+    };
+    // log2 extents of every mode, and the bit offset (log2 stride) of each
+    // mode in each row-major operand
+    std::vector<int32_t> modes;
+    std::vector<int> lext;
+    auto note = [&](const std::vector<int32_t> &m, const std::vector<int64_t> &e) {
+        for (size_t i = 0; i < m.size(); i++) {
+            const int l = lg(e[i]);
+            if (l < 0) return false;
+            if (std::find(modes.begin(), modes.end(), m[i]) == modes.end()) {
+                modes.push_back(m[i]);
+                lext.push_back(l);
+            }
+        }
+        return true;
+    };
+    if (!note(a, ea) || !note(b, eb)) return false;
+    auto lext_of = [&](int32_t m) {
+        return lext[std::find(modes.begin(), modes.end(), m) - modes.begin()];
+    };
+    auto shift_in = [&](const std::vector<int32_t> &op, int32_t m) {
+        int sh = 0;
+        for (size_t i = op.size(); i-- > 0;) {
+            if (op[i] == m) return sh;
+            sh += lext_of(op[i]);
+        }
+        return -1;
+    };
+    int total_a = 0, total_b = 0, total_c = 0;
+    for (auto m : a) total_a += lext_of(m);
+    for (auto m : b) total_b += lext_of(m);
+    for (auto m : c) total_c += lext_of(m);
+    if (total_a > 31 || total_b > 31 || total_c > 31) return false;
+    // group bits (last mode fastest) -> bit positions in operand op
+    auto group = [&](const std::vector<int32_t> &g, const std::vector<int32_t> &op,
+                     FusedBits &fb) {
+        fb.n = 0;
+        for (size_t i = g.size(); i-- > 0;) {
+            const int w = lext_of(g[i]), sh = shift_in(op, g[i]);
+            for (int t = 0; t < w; t++) {
+                if (fb.n >= kFusedMaxBits) return false;
+                fb.pos[fb.n++] = int8_t(sh + t);
+            }
+        }
+        return true;
+    };
+    f = FusedSmall{};
+    if (!group(batch, a, f.a_b) || !group(left, a, f.a_m) || !group(common, a, f.a_k) ||
+        !group(batch, b, f.b_b) || !group(right, b, f.b_n) || !group(common, b, f.b_k) ||
+        !group(batch, c, f.c_b) || !group(left, c, f.c_m) || !group(right, c, f.c_n))
+        return false;
+    f.lb = f.a_b.n;
+    f.lm = f.a_m.n;
+    f.ln = f.b_n.n;
+    f.lk = f.a_k.n;
+    return f.lk <= kFusedMaxLog2K;
+}
+
 int gemm_variants(int kernel)
 {
     switch (kernel) {
@@ -314,6 +382,14 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
             d.fuse_b = plan_split_gather(B, EB, cat(batch, right, common), d.K, d.gb);
     }
 
+    // Small steps on the one-thread-per-output VALU kernel that would need a
+    // permuted operand or result: one fused pass, no copies.
+    if (d.gemm_kernel == kGemmSmall && d.gemm_variant <= 1 && (d.need_a || d.need_b || d.need_c) &&
+        double(d.batch) * double(d.M) * double(d.N) * double(d.K) <= double(1 << 22)) {
+        const auto &common = best_opt == 0 ? common_a : common_b;
+        d.fused_small = plan_fused_small(A, EA, B, EB, C, batch, left, right, common, d.fs);
+    }
+
     layout_workspace(d);
     return d;
 }
@@ -322,6 +398,10 @@ void layout_workspace(ContractionDesc &d)
 {
     size_t off = 0;
     const size_t eb = (size_t)d.elem_bytes;
+    if (d.fused_small) { // reads and writes the tensors in place
+        d.workspace = 0;
+        return;
+    }
     if (d.need_a && !d.fuse_a) {
         d.off_at = off;
         off = align256(off + eb * (size_t)d.size_a);

@@ -35,6 +35,10 @@ struct ContractionDesc {
     bool fuse_a = false, fuse_b = false;
     SplitGather ga, gb;
     PermutePlan pa, pb, pc;
+    // a small step on the one-thread-per-output VALU kernel whose operands
+    // or result need permuting: one fused pass instead (FusedSmall)
+    bool fused_small = false;
+    FusedSmall fs;
     int gemm_kernel = kGemmSmall;
     int gemm_variant = 0; // tile variant of gemm_kernel (GemmChoice)
     // Workspace layout (bytes, 256-aligned offsets).
@@ -65,5 +69,15 @@ void layout_workspace(ContractionDesc &d);
 // tile with the K bits.
 bool plan_split_gather(const std::vector<int32_t> &src, const std::vector<int64_t> &ext,
                        const std::vector<int32_t> &target, int64_t K, SplitGather &g);
+
+// Fused-pass offsets (FusedSmall) of C[batch, left, right] = sum over common
+// of A * B with row-major operands A (modes a, extents ea), B and C (modes
+// c); false when an extent is not a power of two, a group has more than
+// kFusedMaxBits bits, K exceeds 2^kFusedMaxLog2K or an operand 2^31 elements.
+bool plan_fused_small(const std::vector<int32_t> &a, const std::vector<int64_t> &ea,
+                      const std::vector<int32_t> &b, const std::vector<int64_t> &eb,
+                      const std::vector<int32_t> &c, const std::vector<int32_t> &batch,
+                      const std::vector<int32_t> &left, const std::vector<int32_t> &right,
+                      const std::vector<int32_t> &common, FusedSmall &f);
 
 } // namespace jet_hip

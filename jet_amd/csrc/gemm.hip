@@ -242,6 +242,56 @@ __global__ __launch_bounds__(256) void cgemm_small_kernel(GemmArgs g)
     }
 }
 
+// Fused small contraction (FusedSmall): one thread per output element
+// (batch b, row m, column n), the element offsets of A, B and C assembled
+// from their group bits; the k offsets of A and B come from LDS tables that
+// each work-group fills once.  Same summation order and FMA sequence as
+// cgemm_small_kernel.
+__device__ __forceinline__ int64_t fused_offset(const FusedBits &f, int64_t x)
+{
+    int64_t o = 0;
+    for (int i = 0; i < f.n; i++) o |= ((x >> i) & 1) << f.pos[i];
+    return o;
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void fused_small_kernel(const void *Av, const void *Bv,
+                                                          void *Cv, FusedSmall f)
+{
+    using C2 = typename Vec<R>::c;
+    extern __shared__ int32_t koff[]; // [K] offsets in A, then [K] in B
+    const int64_t K = int64_t(1) << f.lk;
+    int32_t *ka = koff, *kb = koff + K;
+    for (int64_t k = threadIdx.x; k < K; k += blockDim.x) {
+        ka[k] = int32_t(fused_offset(f.a_k, k));
+        kb[k] = int32_t(fused_offset(f.b_k, k));
+    }
+    __syncthreads();
+    const C2 *__restrict__ A = static_cast<const C2 *>(Av);
+    const C2 *__restrict__ B = static_cast<const C2 *>(Bv);
+    C2 *__restrict__ C = static_cast<C2 *>(Cv);
+    const int64_t total = int64_t(1) << (f.lb + f.lm + f.ln);
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+         idx += stride) {
+        const int64_t n = idx & ((int64_t(1) << f.ln) - 1);
+        const int64_t m = (idx >> f.ln) & ((int64_t(1) << f.lm) - 1);
+        const int64_t b = idx >> (f.ln + f.lm);
+        const C2 *a = A + (fused_offset(f.a_b, b) | fused_offset(f.a_m, m));
+        const C2 *bb = B + (fused_offset(f.b_b, b) | fused_offset(f.b_n, n));
+        R sr = 0, si = 0;
+        for (int64_t k = 0; k < K; k++) {
+            const C2 x = a[ka[k]], y = bb[kb[k]];
+            sr = fma(x.x, y.x, sr);
+            sr = fma(-x.y, y.y, sr);
+            si = fma(x.x, y.y, si);
+            si = fma(x.y, y.x, si);
+        }
+        C[fused_offset(f.c_b, b) | fused_offset(f.c_m, m) | fused_offset(f.c_n, n)] =
+            C2{sr, si};
+    }
+}
+
 // Short-K products (K <= kShortK) with many outputs are bound by writing C.
 // Column form: a block owns up to kShortRows rows of one batch and 256
 // consecutive columns; each thread keeps its column's K values of B in
@@ -1985,6 +2035,25 @@ void launch_gemm_bf16x4m(const SplitOperand &a, const SplitOperand &b, int64_t K
     prof::Scope ps(prof::kGemmBf16x4m, stream, 8.0 * double(batch) * double(M) * double(N) * K,
                    8.0 * double(batch) * (double(M) * K + double(N) * K + double(M) * double(N)));
     launch_x4m<__bf16>(a, b, C, M, N, ldc, sC, batch, a_bcast, b_bcast, stream, variant);
+}
+
+void launch_fused_small(const FusedSmall &f, int elem_bytes, const void *a, const void *b,
+                        void *c, hipStream_t stream)
+{
+    JH_REQUIRE(f.lk <= kFusedMaxLog2K && f.lb + f.lm + f.ln <= 31, "Fused step out of range.");
+    const int64_t total = int64_t(1) << (f.lb + f.lm + f.ln);
+    const double K = double(int64_t(1) << f.lk);
+    prof::Scope ps(prof::kGemmSmall, stream, 8.0 * double(total) * K,
+                   double(elem_bytes) * double(total));
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(total, int64_t(256)), 4096);
+    const size_t lds = size_t(2) * sizeof(int32_t) << f.lk;
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(fused_small_kernel<float>, dim3(grid), dim3(256), lds, stream, a, b,
+                           c, f);
+    else
+        hipLaunchKernelGGL(fused_small_kernel<double>, dim3(grid), dim3(256), lds, stream, a,
+                           b, c, f);
+    JH_HIP(hipGetLastError());
 }
 
 void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b, int64_t K_true,

@@ -97,6 +97,23 @@ struct SplitGather {
     int8_t wsrc[64] = {}, wrow[64] = {};
 };
 
+// A small contraction of power-of-two extents in one VALU pass that reads A
+// and B and writes C in their own mode orders (no permuted copies, one
+// launch instead of up to four).  Each group of modes (batch, left, right,
+// common) is linearised with its last mode fastest; bit i of a group index
+// sits at bit pos[i] of the element offset in the operand (row-major, every
+// extent a power of two, so the modes' digits occupy disjoint bit ranges).
+constexpr int kFusedMaxBits = 24;  // bits per group
+constexpr int kFusedMaxLog2K = 12; // k offset tables in LDS (2 x 4 KiB entries)
+struct FusedBits {
+    int n = 0;
+    int8_t pos[kFusedMaxBits] = {};
+};
+struct FusedSmall {
+    int lb = 0, lm = 0, ln = 0, lk = 0; // log2 of batch, M, N, K
+    FusedBits a_b, a_m, a_k, b_b, b_n, b_k, c_b, c_m, c_n;
+};
+
 // Split-BF16 operand: planes re hi/lo, im hi/lo[, re+im hi/lo].
 //  * planar (3M kernel, six planes): plane p is [batch][rows_p][k_p] bf16 at
 //    offset p * plane_elems;
@@ -160,6 +177,9 @@ void launch_gemm_bf16x3(const SplitOperand &a, const SplitOperand &b, int64_t K_
                         void *C, int64_t M, int64_t N, int64_t ldc,
                         int64_t sC, int64_t batch, bool a_bcast, bool b_bcast,
                         hipStream_t stream);
+// C = contract(A, B) through FusedSmall offsets, one thread per output.
+void launch_fused_small(const FusedSmall &f, int elem_bytes, const void *a, const void *b,
+                        void *c, hipStream_t stream);
 #endif
 
 } // namespace jet_hip

@@ -141,6 +141,10 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
                      char *ws, hipStream_t s)
 {
     const int eb = d.elem_bytes;
+    if (d.fused_small) {
+        launch_fused_small(d.fs, eb, a, b, c, s);
+        return;
+    }
     const void *At = a;
     const void *Bt = b;
     if (d.need_a && !d.fuse_a) {

@@ -235,3 +235,70 @@ extern "C" int split_gather_targets(int rank, const int32_t *modes, const int64_
     }
     return 0;
 }
+
+// The fused small-contraction pass (plan_fused_small / fused_small_kernel)
+// emulated on the CPU in complex128: the same mode groups as
+// plan_contraction (batch = in A, B and C; left / right = one operand and C;
+// common = A and B only, in A's order), the same group-bit offsets and the
+// same per-output summation over k.  a, b, c interleaved (re, im).  Returns
+// 1 when the fused pass does not apply.
+extern "C" int fused_small_emulate(int ra, const int32_t *ma, const int64_t *ea, int rb,
+                                   const int32_t *mb, const int64_t *eb, int rc,
+                                   const int32_t *mc, const double *a, const double *b,
+                                   double *c)
+{
+    std::vector<int32_t> A(ma, ma + ra), B(mb, mb + rb), C(mc, mc + rc);
+    std::vector<int64_t> EA(ea, ea + ra), EB(eb, eb + rb);
+    auto has = [](const std::vector<int32_t> &v, int32_t m) {
+        for (auto x : v)
+            if (x == m) return true;
+        return false;
+    };
+    std::vector<int32_t> batch, left, right, common;
+    for (auto m : C) {
+        if (has(A, m) && has(B, m))
+            batch.push_back(m);
+        else if (has(A, m))
+            left.push_back(m);
+        else
+            right.push_back(m);
+    }
+    for (auto m : A)
+        if (has(B, m) && !has(C, m)) common.push_back(m);
+    FusedSmall f;
+    if (!plan_fused_small(A, EA, B, EB, C, batch, left, right, common, f)) return 1;
+    auto off = [](const FusedBits &fb, int64_t x) {
+        int64_t o = 0;
+        for (int i = 0; i < fb.n; i++) o |= ((x >> i) & 1) << fb.pos[i];
+        return o;
+    };
+    const int64_t total = int64_t(1) << (f.lb + f.lm + f.ln), K = int64_t(1) << f.lk;
+    for (int64_t idx = 0; idx < total; idx++) {
+        const int64_t n = idx & ((int64_t(1) << f.ln) - 1);
+        const int64_t m = (idx >> f.ln) & ((int64_t(1) << f.lm) - 1);
+        const int64_t bb = idx >> (f.ln + f.lm);
+        const int64_t oa = off(f.a_b, bb) | off(f.a_m, m), ob = off(f.b_b, bb) | off(f.b_n, n);
+        double sr = 0, si = 0;
+        for (int64_t k = 0; k < K; k++) {
+            const int64_t ia = oa | off(f.a_k, k), ib = ob | off(f.b_k, k);
+            const double xr = a[2 * ia], xi = a[2 * ia + 1], yr = b[2 * ib], yi = b[2 * ib + 1];
+            sr += xr * yr - xi * yi;
+            si += xr * yi + xi * yr;
+        }
+        const int64_t oc = off(f.c_b, bb) | off(f.c_m, m) | off(f.c_n, n);
+        c[2 * oc] = sr;
+        c[2 * oc + 1] = si;
+    }
+    return 0;
+}
+
+// Whether plan_contraction (compiled-in GEMM table) fuses this contraction
+// of complex64 operands in FP32 (mode 0): 1 fused, 0 not.
+extern "C" int contraction_is_fused(int ra, const int32_t *ma, const int64_t *ea, int rb,
+                                    const int32_t *mb, const int64_t *eb, int rc,
+                                    const int32_t *mc, size_t *workspace)
+{
+    const ContractionDesc d = plan_contraction(8, 0, 0, ra, ma, ea, rb, mb, eb, rc, mc);
+    *workspace = d.workspace;
+    return d.fused_small ? 1 : 0;
+}
