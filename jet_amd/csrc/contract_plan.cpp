@@ -382,12 +382,16 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
             d.fuse_b = plan_split_gather(B, EB, cat(batch, right, common), d.K, d.gb);
     }
 
-    // Small steps on the one-thread-per-output VALU kernel that would need a
-    // permuted operand or result: one fused pass, no copies.
-    if (d.gemm_kernel == kGemmSmall && d.gemm_variant <= 1 && (d.need_a || d.need_b || d.need_c) &&
-        double(d.batch) * double(d.M) * double(d.N) * double(d.K) <= double(1 << 22)) {
+    // Small steps on the one-thread-per-output or the one-block-per-output
+    // VALU kernel that would need a permuted operand or result: one fused
+    // pass of the same form, no copies.
+    const double work = double(d.batch) * double(d.M) * double(d.N) * double(d.K);
+    const bool small = d.gemm_kernel == kGemmSmall && d.gemm_variant <= 1 && work <= double(1 << 22);
+    const bool red = d.gemm_kernel == kGemmSmallRed && work <= double(1 << 26);
+    if ((small || red) && (d.need_a || d.need_b || d.need_c)) {
         const auto &common = best_opt == 0 ? common_a : common_b;
         d.fused_small = plan_fused_small(A, EA, B, EB, C, batch, left, right, common, d.fs);
+        d.fs.red = red ? 1 : 0;
     }
 
     layout_workspace(d);

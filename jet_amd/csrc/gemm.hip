@@ -292,6 +292,60 @@ __global__ __launch_bounds__(256) void fused_small_kernel(const void *Av, const 
     }
 }
 
+// The one-work-group-per-output form of the fused pass (few outputs, long
+// K): threads take k = t, t + 256, ..., the partial sums are reduced across
+// the wave with shuffles and across the four waves in LDS.
+template <class R>
+__global__ __launch_bounds__(256) void fused_small_red_kernel(const void *Av, const void *Bv,
+                                                              void *Cv, FusedSmall f)
+{
+    using C2 = typename Vec<R>::c;
+    extern __shared__ int32_t koff[]; // [K] offsets in A, then [K] in B
+    __shared__ R red_s[2][4];
+    const int64_t K = int64_t(1) << f.lk;
+    int32_t *ka = koff, *kb = koff + K;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int64_t k = tid; k < K; k += blockDim.x) {
+        ka[k] = int32_t(fused_offset(f.a_k, k));
+        kb[k] = int32_t(fused_offset(f.b_k, k));
+    }
+    __syncthreads();
+    const C2 *__restrict__ A = static_cast<const C2 *>(Av);
+    const C2 *__restrict__ B = static_cast<const C2 *>(Bv);
+    C2 *__restrict__ C = static_cast<C2 *>(Cv);
+    const int64_t total = int64_t(1) << (f.lb + f.lm + f.ln);
+    for (int64_t idx = blockIdx.x; idx < total; idx += gridDim.x) {
+        const int64_t n = idx & ((int64_t(1) << f.ln) - 1);
+        const int64_t m = (idx >> f.ln) & ((int64_t(1) << f.lm) - 1);
+        const int64_t b = idx >> (f.ln + f.lm);
+        const C2 *a = A + (fused_offset(f.a_b, b) | fused_offset(f.a_m, m));
+        const C2 *bb = B + (fused_offset(f.b_b, b) | fused_offset(f.b_n, n));
+        R sr = 0, si = 0;
+        for (int64_t k = tid; k < K; k += blockDim.x) {
+            const C2 x = a[ka[k]], y = bb[kb[k]];
+            sr = fma(x.x, y.x, sr);
+            sr = fma(-x.y, y.y, sr);
+            si = fma(x.x, y.y, si);
+            si = fma(x.y, y.x, si);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off);
+            si += __shfl_xor(si, off);
+        }
+        if (lane == 0) {
+            red_s[0][wave] = sr;
+            red_s[1][wave] = si;
+        }
+        __syncthreads();
+        if (tid == 0)
+            C[fused_offset(f.c_b, b) | fused_offset(f.c_m, m) | fused_offset(f.c_n, n)] =
+                C2{(red_s[0][0] + red_s[0][1]) + (red_s[0][2] + red_s[0][3]),
+                   (red_s[1][0] + red_s[1][1]) + (red_s[1][2] + red_s[1][3])};
+        __syncthreads();
+    }
+}
+
 // Short-K products (K <= kShortK) with many outputs are bound by writing C.
 // Column form: a block owns up to kShortRows rows of one batch and 256
 // consecutive columns; each thread keeps its column's K values of B in
@@ -2045,8 +2099,19 @@ void launch_fused_small(const FusedSmall &f, int elem_bytes, const void *a, cons
     const double K = double(int64_t(1) << f.lk);
     prof::Scope ps(prof::kGemmSmall, stream, 8.0 * double(total) * K,
                    double(elem_bytes) * double(total));
-    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(total, int64_t(256)), 4096);
     const size_t lds = size_t(2) * sizeof(int32_t) << f.lk;
+    if (f.red) {
+        const unsigned grid = (unsigned)std::min<int64_t>(total, 65536);
+        if (elem_bytes == 8)
+            hipLaunchKernelGGL(fused_small_red_kernel<float>, dim3(grid), dim3(256), lds, stream,
+                               a, b, c, f);
+        else
+            hipLaunchKernelGGL(fused_small_red_kernel<double>, dim3(grid), dim3(256), lds,
+                               stream, a, b, c, f);
+        JH_HIP(hipGetLastError());
+        return;
+    }
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(total, int64_t(256)), 4096);
     if (elem_bytes == 8)
         hipLaunchKernelGGL(fused_small_kernel<float>, dim3(grid), dim3(256), lds, stream, a, b,
                            c, f);

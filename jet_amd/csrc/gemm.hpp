@@ -111,6 +111,7 @@ struct FusedBits {
 };
 struct FusedSmall {
     int lb = 0, lm = 0, ln = 0, lk = 0; // log2 of batch, M, N, K
+    int red = 0; // 1: one work-group per output, k split over its threads
     FusedBits a_b, a_m, a_k, b_b, b_n, b_k, c_b, c_m, c_n;
 };
 

@@ -75,6 +75,11 @@ def _is_fused(L, ma, ea, mb, eb, mc):
 
 def test_planner_fuses_small_permuted_steps():
     L = _check_lib()
+    # few outputs, long K (the one-block-per-output kernel): 2 x 4 outputs,
+    # K = 2^10, scrambled
+    r, ws = _is_fused(L, [3, 10, 0, 5, 11, 1, 7, 9, 2, 8, 4, 6], [2] * 12,
+                      [6, 12, 2, 9, 4, 13, 8, 1, 3, 7, 5, 0], [2] * 12, [11, 13, 12, 10])
+    assert r == 1 and ws == 0
     # bond-2 rank-6 x rank-5, two shared modes, scrambled orders: tiny, permuted
     r, ws = _is_fused(L, [0, 3, 1, 4, 2, 5], [2] * 6, [5, 6, 4, 7, 8], [2] * 5,
                       [8, 0, 6, 1, 2, 3, 7])

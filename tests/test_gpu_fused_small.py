@@ -37,3 +37,36 @@ def test_fused_small_contractions(ctx, dtype, prec, tol):
         assert rel_err(got, want) < tol, (case, info, rel_err(got, want))
     if dtype == np.complex64:
         assert fused >= 6, fused   # a good share of these small steps is fused
+
+
+@pytest.mark.parametrize("dtype,prec,tol", [(np.complex64, nat.PREC_FP32, 1e-5),
+                                            (np.complex128, nat.PREC_FP64, 1e-12)])
+def test_fused_small_reductions(ctx, dtype, prec, tol):
+    """Few outputs and long K (2^9-2^12, the one-work-group-per-output form),
+    scrambled orders, against einsum."""
+    L = _check_lib()
+    rng = np.random.default_rng(37)
+    fused = 0
+    for case in range(6):
+        nk = int(rng.integers(9, 13))
+        nl, nr, nb = int(rng.integers(1, 3)), int(rng.integers(0, 3)), int(rng.integers(0, 2))
+        ids = [int(x) for x in rng.permutation(nk + nl + nr + nb)]
+        com, lef = ids[:nk], ids[nk:nk + nl]
+        rig, bat = ids[nk + nl:nk + nl + nr], ids[nk + nl + nr:]
+        ma = [int(x) for x in rng.permutation(bat + lef + com)]
+        mb = [int(x) for x in rng.permutation(bat + rig + com)]
+        mc = [int(x) for x in rng.permutation(bat + lef + rig)]
+        sa, sb = [2] * len(ma), [2] * len(mb)
+        if dtype == np.complex64:
+            fused += _is_fused(L, ma, sa, mb, sb, mc)[0]
+        a = rand_c(rng, 2 ** len(ma), dtype)
+        b = rand_c(rng, 2 ** len(mb), dtype)
+        say(f"fused reduction {np.dtype(dtype).name} case {case}: K = 2^{nk}")
+        got, info = contract(ctx, a, ma, sa, b, mb, sb, mc, precision=prec, workspace=True)
+        sub = lambda ms: "".join(LETTERS[m] for m in ms)
+        want = np.einsum(f"{sub(ma)},{sub(mb)}->{sub(mc)}",
+                         a.astype(np.complex128).reshape(sa),
+                         b.astype(np.complex128).reshape(sb)).reshape(-1)
+        assert rel_err(got, want) < tol, (case, info, rel_err(got, want))
+    if dtype == np.complex64:
+        assert fused >= 3, fused
