@@ -281,6 +281,8 @@ def run_c1(args):
     gemm_ms = gemm.get("ms", 0.0) / max(1, gemm.get("launches", 1))
     gemm_flops = gemm.get("flops", 0.0) / max(1, gemm.get("launches", 1))
     achieved = gemm_flops / (gemm_ms * 1e-3) if gemm_ms else 0.0
+    loop = mfma_loop_ceiling(ctx)
+    log(f"c1: FP16 MFMA loop {loop[0] / 1e12:.1f} TFLOP/s at {loop[1]:.3f} GHz")
     result = {
         "metric": METRIC,
         "value": round(flops / (ms16 * 1e-3) / 1e12, 3),
@@ -309,6 +311,8 @@ def run_c1(args):
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK["bf16"], 4),
                      "hw_16bit_mfma_frac": round(achieved * SPLIT_MODES[args.split][3] /
                                                  PEAK["bf16"], 4),
+                     "mfma_loop": mfma_loop_fields(achieved * SPLIT_MODES[args.split][3],
+                                                   *loop),
                      "traffic": profiled_traffic("c1_split_contraction.md", kname)[0],
                      "traffic_source": ("profiles/c1_split_contraction.md: rocprofv3 --pmc "
                                         "FETCH_SIZE / WRITE_SIZE, read at commit "
@@ -364,6 +368,31 @@ C2_PROFILE = {"refined": "c2_sliced_amplitude.md", "min-fill": "c2_min_fill_path
 def dname_of(prof):
     """The kernel family with the most device time in an event profile."""
     return max(prof.items(), key=lambda kv: kv[1]["ms"])[0]
+
+
+def mfma_loop_ceiling(ctx, rounds=200000, reps=3):
+    """The library's FP16 MFMA rate loop (v_mfma_f32_16x16x32_f16 on register
+    operands, no memory traffic, every CU): the FP16 MFMA TFLOP/s the chip
+    sustains under a full MFMA load and the shader clock it holds meanwhile
+    -- the practical ceiling of the split GEMMs' FP16 MFMA work (about 0.3 s
+    on MI355X)."""
+    import ctypes
+    from jet_amd import _native as nat
+    tf, ghz = ctypes.c_double(), ctypes.c_double()
+    nat.check(nat.lib().jet_hip_mfma_ceiling(ctx.handle, rounds, reps, ctypes.byref(tf),
+                                             ctypes.byref(ghz)))
+    return tf.value * 1e12, ghz.value
+
+
+def mfma_loop_fields(achieved_hw, loop_flops, loop_ghz):
+    """Roofline fields relating a split GEMM's FP16 MFMA work rate to the
+    measured loop ceiling."""
+    return {"fp16_tflops": round(loop_flops / 1e12, 1), "clock_ghz": round(loop_ghz, 3),
+            "hw_16bit_frac_of_loop": round(achieved_hw / loop_flops, 4),
+            "note": "FP16 MFMA rate of the library's register-only v_mfma_f32_16x16x32_f16 "
+                    "loop over every CU, measured in this run: the rate the chip sustains "
+                    "under a full MFMA load (it clocks down from the 2.4 GHz of the nominal "
+                    "peak)"}
 
 
 def hbm_copy_ceiling(ctx, gib=2, reps=10):
@@ -573,6 +602,8 @@ def run_c2(args):
     sps32 = steps32 * n_batch / (ms32 * 1e-3)
     traffic, traffic_commit = profiled_traffic(C2_PROFILE[args.c2_plan], dname_of(prof16))
     copy_bps = hbm_copy_ceiling(ctx)
+    loop = mfma_loop_ceiling(ctx)
+    log(f"c2: FP16 MFMA loop {loop[0] / 1e12:.1f} TFLOP/s at {loop[1]:.3f} GHz")
     perm = prof16.get("permute_bits")
     perm_bps = perm["bytes"] / (perm["ms"] * 1e-3) if perm and perm["ms"] > 0 else None
     split = prof16.get("split_f16")
@@ -646,6 +677,8 @@ def run_c2(args):
                      "peak": peak / scale, "unit": unit, "frac": round(achieved / peak, 4),
                      "hw_16bit_mfma_frac": (round(achieved * SPLIT_MODES[args.split][3] / peak, 4)
                                             if dname == SPLIT_MODES[args.split][2] else None),
+                     "mfma_loop": (mfma_loop_fields(achieved * SPLIT_MODES[args.split][3], *loop)
+                                   if dname == SPLIT_MODES[args.split][2] else None),
                      "algorithmic_bytes": round(dk["bytes"] / dk["launches"]),
                      "traffic": traffic,
                      "traffic_source": (f"profiles/{C2_PROFILE[args.c2_plan]}: rocprofv3 --pmc "

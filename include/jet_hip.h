@@ -362,6 +362,22 @@ int jet_hip_event_record(jet_hip_ctx_t ctx, void *event, void *stream);
 int jet_hip_event_elapsed_ms(void *start, void *stop, float *ms);
 int jet_hip_event_destroy(void *event);
 
+/* ---- sustained FP16 MFMA rate (no counterpart in the reference: a
+ *      measurement of this device).  A loop of v_mfma_f32_16x16x32_f16 --
+ *      the instruction of the split GEMMs -- on register operands (uniform
+ *      in (-1, 1) with full mantissas, like the split operands: the clock
+ *      the chip holds depends on the operand bits), with no memory
+ *      traffic, on four work-groups of four waves per CU: one warm-up
+ *      and `reps` timed launches of `rounds` (a positive multiple of 4)
+ *      rounds of eight MFMAs per wave, on the context stream.  *tflops: FP16
+ *      MFMA TFLOP/s over the timed launches; *clock_ghz: the mean shader
+ *      clock of the last launch's waves (clock-counter deltas over their
+ *      constant-rate timer deltas).  The chip clocks down under a full MFMA
+ *      load, so this rate, not the nominal peak at the 2.4 GHz peak clock,
+ *      is the practical ceiling of MFMA-bound kernels.  Allocates.        */
+int jet_hip_mfma_ceiling(jet_hip_ctx_t ctx, int rounds, int reps, double *tflops,
+                         double *clock_ghz);
+
 /* ---- per-kernel profiler (no counterpart in the reference, which has no
  *      tracer).  When enabled, each kernel launch is bracketed by events on
  *      its stream; jet_hip_profiler_read waits for them and returns, for each

@@ -39,7 +39,8 @@ EXPORTS = [
     "jet_hip_contract", "jet_hip_contract_noalloc", "jet_hip_add", "jet_hip_slice", "jet_hip_fill_random",
     "jet_hip_reduce_compensated", "jet_hip_convert", "jet_hip_event_create",
     "jet_hip_event_record", "jet_hip_event_elapsed_ms", "jet_hip_event_destroy",
-    "jet_hip_profiler_enable", "jet_hip_profiler_reset", "jet_hip_profiler_read",
+    "jet_hip_mfma_ceiling", "jet_hip_profiler_enable", "jet_hip_profiler_reset",
+    "jet_hip_profiler_read",
     "jet_hip_profiler_kernel_name", "jet_hip_network_create", "jet_hip_network_estimate", "jet_hip_network_info",
     "jet_hip_network_step_info", "jet_hip_network_execute", "jet_hip_network_capture",
     "jet_hip_network_launch", "jet_hip_network_destroy", "jet_hip_batch_capture",
@@ -143,6 +144,8 @@ def _declare(L):
         "jet_hip_dag_value": [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int),
                               _i32p, _i64p],
         "jet_hip_batch_destroy": [_vp],
+        "jet_hip_mfma_ceiling": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_double)],
         "jet_hip_profiler_enable": [ctypes.c_int],
         "jet_hip_profiler_reset": [],
         "jet_hip_profiler_read": [ctypes.c_int, _i64p, ctypes.POINTER(ctypes.c_double),

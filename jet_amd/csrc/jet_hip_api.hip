@@ -831,6 +831,63 @@ int jet_hip_convert(jet_hip_ctx_t ctx, jet_hip_dtype_t src_dtype, jet_hip_dtype_
     });
 }
 
+int jet_hip_mfma_ceiling(jet_hip_ctx_t ctx, int rounds, int reps, double *tflops,
+                         double *clock_ghz)
+{
+    return guarded([&] {
+        set_device(ctx);
+        JH_REQUIRE(tflops != nullptr && clock_ghz != nullptr, "Null output pointer.");
+        JH_REQUIRE(rounds > 0 && rounds % 4 == 0 && reps > 0,
+                   "rounds must be a positive multiple of 4 and reps positive.");
+        int device = 0, cus = 0, wall_khz = 0;
+        JH_HIP(hipGetDevice(&device));
+        JH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        JH_HIP(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device));
+        JH_REQUIRE(cus > 0 && wall_khz > 0, "Device attributes unavailable.");
+        const int blocks = 4 * cus; // four work-groups of four waves per CU
+        hipStream_t s = pick_stream(ctx, nullptr);
+        float *sink = nullptr;
+        long long *clocks = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        std::vector<long long> host(size_t(8) * blocks);
+        auto release = [&] {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (sink) (void)hipFree(sink);
+            if (clocks) (void)hipFree(clocks);
+        };
+        try {
+            JH_HIP(hipMalloc(&sink, sizeof(float) * 256 * size_t(blocks)));
+            JH_HIP(hipMalloc(&clocks, sizeof(long long) * host.size()));
+            JH_HIP(hipEventCreate(&e0));
+            JH_HIP(hipEventCreate(&e1));
+            launch_mfma_loop(sink, clocks, blocks, rounds, s); // warm-up
+            JH_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; r++) launch_mfma_loop(sink, clocks, blocks, rounds, s);
+            JH_HIP(hipEventRecord(e1, s));
+            JH_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            JH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            JH_HIP(hipMemcpy(host.data(), clocks, sizeof(long long) * host.size(),
+                             hipMemcpyDeviceToHost));
+            double shader = 0.0, wall = 0.0;
+            for (int w = 0; w < 4 * blocks; w++) {
+                shader += double(host[2 * w]);
+                wall += double(host[2 * w + 1]);
+            }
+            // 16 * 16 * 32 multiply-adds per MFMA, eight per round and wave
+            const double flop = 2.0 * 16 * 16 * 32 * 8.0 * double(rounds) * 4.0 * blocks * reps;
+            *tflops = flop / (double(ms) * 1e-3) / 1e12;
+            *clock_ghz = wall > 0 ? shader / (wall / (double(wall_khz) * 1e3)) / 1e9 : 0.0;
+        }
+        catch (...) {
+            release();
+            throw;
+        }
+        release();
+    });
+}
+
 int jet_hip_event_create(jet_hip_ctx_t ctx, void **event)
 {
     return guarded([&] {

@@ -1,5 +1,5 @@
-// misc.hip -- elementwise add, seeded fill and the FP64 compensated
-// reduction (gfx950).
+// misc.hip -- elementwise add, seeded fill, the FP64 compensated
+// reduction and the FP16 MFMA rate loop (gfx950).
 #include "common.hpp"
 #include "misc.hpp"
 #include "profiler.hpp"
@@ -102,6 +102,55 @@ __global__ __launch_bounds__(256) void convert_kernel(const S *__restrict__ in,
 
 int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
 
+
+typedef _Float16 mfma_h8 __attribute__((ext_vector_type(8)));
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+// FP16 MFMA rate loop: every wave runs `rounds` (a multiple of 4) rounds of
+// eight independent v_mfma_f32_16x16x32_f16 accumulator chains on operands
+// held in registers -- the instruction of the split GEMMs, with no memory
+// traffic -- and records its shader-clock and constant-rate-timer deltas.
+__global__ __launch_bounds__(256) void mfma_loop_kernel(float *__restrict__ sink,
+                                                        long long *__restrict__ clocks,
+                                                        int rounds)
+{
+    const int tid = threadIdx.x;
+    // Uniform operands in (-1, 1) with full mantissas, like the split
+    // operands' planes: the chip's power draw, hence the clock it holds,
+    // depends on the operand bits (low-entropy operands run up to ~20%
+    // faster).
+    mfma_h8 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const uint64_t c = (uint64_t(blockIdx.x) * 256 + tid) * 64 + i * 8 + e;
+            a[i][e] = (_Float16)(float(2.0 * counter_uniform(11, c) - 1.0));
+            b[i][e] = (_Float16)(float(2.0 * counter_uniform(13, c) - 1.0));
+        }
+    mfma_f4 acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    const long long c0 = clock64(), w0 = wall_clock64();
+    for (int it = 0; it < rounds; it += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i & 3], b[(i + u) & 3], acc[i],
+                                                                0, 0, 0);
+    }
+    const long long c1 = clock64(), w1 = wall_clock64();
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s += acc[i][0] + acc[i][3];
+    sink[blockIdx.x * 256 + tid] = s;
+    if ((tid & 63) == 0) {
+        const int w = blockIdx.x * 4 + (tid >> 6);
+        clocks[2 * w] = c1 - c0;
+        clocks[2 * w + 1] = w1 - w0;
+    }
+}
 } // namespace
 
 void launch_add(int elem_bytes, const void *a, const void *b, void *c, int64_t n,
@@ -161,6 +210,13 @@ void launch_reduce(int elem_bytes, const void *x, int64_t n, double *out,
     else
         hipLaunchKernelGGL(reduce_kernel<double2>, dim3(1), dim3(1024), 0, stream,
                            static_cast<const double2 *>(x), n, out);
+    JH_HIP(hipGetLastError());
+}
+
+void launch_mfma_loop(float *sink, long long *clocks, int blocks, int rounds, hipStream_t stream)
+{
+    JH_REQUIRE(blocks > 0 && rounds > 0 && rounds % 4 == 0, "MFMA loop: bad launch shape.");
+    hipLaunchKernelGGL(mfma_loop_kernel, dim3(blocks), dim3(256), 0, stream, sink, clocks, rounds);
     JH_HIP(hipGetLastError());
 }
 

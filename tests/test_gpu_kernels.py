@@ -582,3 +582,18 @@ def test_split_f16_unaligned_operands(ctx, builtin_rule):
     finally:
         nat.lib().jet_hip_plan_destroy(plan)
     assert np.array_equal(got, want)
+
+
+def test_mfma_ceiling_loop(ctx):
+    """The FP16 MFMA rate loop reports a positive rate below the nominal
+    2.5 PFLOP/s dense FP16 peak and a shader clock in the chip's range, and
+    refuses round counts that are not positive multiples of 4."""
+    tf, ghz = ctypes.c_double(), ctypes.c_double()
+    say("mfma ceiling: 4000 rounds, 1 rep")
+    nat.check(nat.lib().jet_hip_mfma_ceiling(ctx.handle, 4000, 1, ctypes.byref(tf),
+                                             ctypes.byref(ghz)))
+    assert 100.0 < tf.value < 2600.0, tf.value
+    assert 0.3 < ghz.value < 3.0, ghz.value
+    assert nat.lib().jet_hip_mfma_ceiling(ctx.handle, 6, 1, ctypes.byref(tf),
+                                          ctypes.byref(ghz)) != 0
+    assert "multiple of 4" in nat.lib().jet_hip_last_error().decode()
