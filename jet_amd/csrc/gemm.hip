@@ -789,7 +789,8 @@ __global__ __launch_bounds__(256) void split_f16_block_rows_kernel(
 //      next tile are issued into registers before the current tile is
 //      split (19% faster on a 2^31-element operand);
 //   2. each thread takes its 8-value chunks of target rows from LDS into
-//      registers and folds them into the row maxima (LDS atomics);
+//      registers and folds them into the row maxima (shuffles within the
+//      wave for K <= 512; LDS atomics across waves for longer rows);
 //   3. the planes and scales are written from registers as
 //      split_f16_rows_kernel writes them.
 // Every per-element offset is computed once per thread, outside the tile
@@ -814,6 +815,10 @@ __global__ __launch_bounds__(256) void split_f16_gather_kernel(
     const int tid = threadIdx.x;
     const int tsz = 1 << g.tb;
     const int kq_bits = g.log2k - 3;
+    // K <= 512: a target row's 8-value chunks lie in one wave, and the
+    // shuffles alone give every lane its row maximum (no LDS atomics and
+    // no barrier between the maxima and the split)
+    const bool wave_rows = kq_bits <= 6;
     const int nrows = 1 << g.xr;
     // v-bit -> u-bit
     int v2u[16];
@@ -890,7 +895,7 @@ __global__ __launch_bounds__(256) void split_f16_gather_kernel(
         int64_t rbase = 0;
         for (int j = 0; j < g.nw; j++)
             if ((w >> j) & 1) rbase += int64_t(1) << g.wrow[j];
-        if (tid < nrows) rowmax[tid] = 0u;
+        if (!wave_rows && tid < nrows) rowmax[tid] = 0u;
 #pragma unroll
         for (int e = 0; e < kGatherPairs; e++)
             if (2 * (tid + e * 256) < tsz) {
@@ -900,6 +905,7 @@ __global__ __launch_bounds__(256) void split_f16_gather_kernel(
         if (w + gridDim.x < ntiles) fetch(w + gridDim.x);
         __syncthreads();
         float2 x[kGatherChunks][8];
+        uint32_t rm[kGatherChunks];
 #pragma unroll
         for (int c = 0; c < kGatherChunks; c++) {
             uint32_t m = 0u;
@@ -911,17 +917,19 @@ __global__ __launch_bounds__(256) void split_f16_gather_kernel(
                                __float_as_uint(x[c][e].y) & 0x7fffffffu));
             }
             // lanes of one row are consecutive: reduce within each group of
-            // min(64, chunks per row) lanes, then one LDS atomic per group
+            // min(64, chunks per row) lanes; rows longer than a wave then
+            // combine their groups with one LDS atomic per group
             const int grp = kq_bits < 6 ? (1 << kq_bits) : 64;
             for (int off = grp >> 1; off > 0; off >>= 1)
                 m = max(m, (uint32_t)__shfl_xor((int)m, off));
-            if (live && (tid & (grp - 1)) == 0) atomicMax(&rowmax[crow[c]], m);
+            rm[c] = m;
+            if (!wave_rows && live && (tid & (grp - 1)) == 0) atomicMax(&rowmax[crow[c]], m);
         }
-        __syncthreads();
+        if (!wave_rows) __syncthreads();
 #pragma unroll
         for (int c = 0; c < kGatherChunks; c++) {
             if (8 * (tid + c * 256) >= tsz) continue;
-            const int sh = row_shift(rowmax[crow[c]]);
+            const int sh = row_shift(wave_rows ? rm[c] : rowmax[crow[c]]);
             const float sc = __builtin_ldexpf(1.f, sh);
             const int64_t row = rbase + croff[c];
             const int kq = ckq[c];

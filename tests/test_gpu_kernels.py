@@ -487,6 +487,34 @@ def test_split_f16_exact_integers(ctx, builtin_rule, K):
     assert np.array_equal(got.reshape(M, N), want)
 
 
+@pytest.mark.parametrize("kbits", [6, 8, 9, 10])
+def test_split_f16_gathered_exact_integers(ctx, builtin_rule, kbits):
+    """Exact-split operands (small integers times a power of two per row) in
+    a scrambled mode order whose lowest source bits are K bits, so that the
+    permutation is folded into the split (gathered tiles): for K = 64, 256
+    and 512 the row maxima come from shuffles within a wave, for K = 1024
+    from LDS atomics across waves.  The product is exact, bit for bit."""
+    rng = np.random.default_rng(300 + kbits)
+    common = list(range(kbits))
+    left = list(range(20, 28))
+    right = list(range(40, 47))
+    M, N, K = 2 ** len(left), 2 ** len(right), 2 ** kbits
+    ints = lambda n, m: (rng.integers(-8, 9, (n, m)) + 1j * rng.integers(-8, 9, (n, m)))
+    at = (ints(M, K) * np.ldexp(1.0, rng.integers(-20, 21, M))[:, None]).astype(np.complex64)
+    bt = (ints(N, K) * np.ldexp(1.0, rng.integers(-20, 21, N))[:, None]).astype(np.complex64)
+    lp = [int(x) for x in rng.permutation(left)]
+    ma = lp[:4] + [common[0]] + lp[4:] + common[1:]     # lowest source bits: K modes
+    a = at.reshape([2] * (len(left) + kbits)).transpose(
+        [(left + common).index(m) for m in ma]).reshape(-1)
+    mb = right + common
+    say(f"split-f16 gathered exact integers K={K} A{ma}")
+    got, info = contract(ctx, np.ascontiguousarray(a), ma, [2] * len(ma), bt.reshape(-1), mb,
+                         [2] * len(mb), left + right, precision=nat.PREC_F16X3_4M)
+    assert info["kernel"] == "f16x3_4m_mfma", info
+    want = (at.astype(np.complex128) @ bt.astype(np.complex128).T).astype(np.complex64)
+    assert np.array_equal(got.reshape(M, N), want)
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
 def test_split_f16_gathered_operands(ctx, builtin_rule, seed):
     """Split-FP16 contractions of bond-2 tensors in scrambled mode orders:
