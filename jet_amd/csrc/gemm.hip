@@ -2109,7 +2109,11 @@ void launch_fused_small(const FusedSmall &f, int elem_bytes, const void *a, cons
                    double(elem_bytes) * double(total));
     const size_t lds = size_t(2) * sizeof(int32_t) << f.lk;
     if (f.red) {
-        const unsigned grid = (unsigned)std::min<int64_t>(total, 65536);
+        // Eight work-groups per CU at most: each builds its k offset tables
+        // once and walks many outputs (one work-group per output rebuilt the
+        // tables for every output, about half of the time of a many-output
+        // step).
+        const unsigned grid = (unsigned)std::min<int64_t>(total, 2048);
         if (elem_bytes == 8)
             hipLaunchKernelGGL(fused_small_red_kernel<float>, dim3(grid), dim3(256), lds, stream,
                                a, b, c, f);
