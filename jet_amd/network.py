@@ -116,7 +116,8 @@ class DeviceNetwork:
                     kernel=nat.GEMM_KERNEL_NAMES[kern.value], permutes=perm.value)
 
     def _ptrs(self, leaves):
-        assert len(leaves) == self.n_leaves
+        if len(leaves) != self.n_leaves:
+            raise JetError(f"Expected {self.n_leaves} leaf pointers, got {len(leaves)}.")
         return (ctypes.c_void_p * self.n_leaves)(*[int(x) for x in leaves])
 
     def execute(self, leaf_ptrs: Sequence[int], out_ptr: int, stream=None) -> None:
@@ -192,7 +193,8 @@ class DeviceDag:
         self._graph_key = None
 
     def _ptrs(self, leaves):
-        assert len(leaves) == self.n_leaves
+        if len(leaves) != self.n_leaves:
+            raise JetError(f"Expected {self.n_leaves} leaf pointers, got {len(leaves)}.")
         return (ctypes.c_void_p * self.n_leaves)(*[int(x) for x in leaves])
 
     def execute(self, leaf_ptrs: Sequence[int], stream=None) -> None:
@@ -283,12 +285,13 @@ class SlicedContraction:
     def __init__(self, indices, shapes, data, path, sliced: Sequence[str],
                  batch: Optional[Sequence[str]] = None, dtype=np.complex64,
                  precision: Optional[str] = None, ctx=None, use_graph: bool = True):
+        sliced = list(sliced)
+        batch = list(sliced if batch is None else batch)
+        if not set(batch) <= set(sliced):
+            raise JetError("Batch indices must be sliced indices.")
         self.ctx = ctx or nat.default_context()
         self.dtype = np.dtype(dtype)
         self.code = nat.C64 if self.dtype == np.complex64 else nat.C128
-        sliced = list(sliced)
-        batch = list(sliced if batch is None else batch)
-        assert set(batch) <= set(sliced)
         self.loop = [l for l in sliced if l not in batch]
         self.batch = [l for l in sliced if l in batch]
         # Slice order of the results: loop indices first, then batch indices.

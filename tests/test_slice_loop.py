@@ -25,6 +25,15 @@ def test_execution_slot_rejects_bad_arguments():
         execution_slot(0, 0)
 
 
+def test_batch_indices_must_be_sliced():
+    """Batch modes are a subset of the sliced indices; the check raises
+    before any device work."""
+    from jet_amd.network import SlicedContraction
+    with pytest.raises(JetError, match="sliced indices"):
+        SlicedContraction([["a", "b"], ["b"]], [[2, 2], [2]], [None, None], [1, 0],
+                          sliced=["b"], batch=["a"])
+
+
 def test_execution_slot_writes_stay_in_bounds():
     """Every byte an iteration writes lies in the results vector, for 5
     warm-up + 20 timed iterations (bench.py --warmup 5 --steps 20) on a
