@@ -372,14 +372,60 @@ ContractionDesc plan_contraction(int elem_bytes, int mode, int layout,
     d.gemm_kernel = choice.kernel;
     d.gemm_variant = choice.variant;
     d.flops = 8.0 * double(d.batch) * double(d.M) * double(d.N) * double(d.K);
+    std::vector<int32_t> gemm_batch = batch;
+    std::vector<int32_t> gemm_common = best_opt == 0 ? common_a : common_b;
+    {
+        // Many outputs and a long K on a VALU kernel (the split-K and
+        // one-block-per-output forms are built for few outputs): split K
+        // into chunks that become leading batch modes of an MFMA GEMM, then
+        // sum the chunks (M and N at least 64).  The chunk modes are the leading common modes
+        // (the slowest-varying K digits), about 512 output tiles in all,
+        // and at least 2048 k per chunk.
+        const bool valu = d.gemm_kernel == kGemmSplitK || d.gemm_kernel == kGemmSmallRed;
+        const bool mfma_mode = mode == 0 || mode == 2 || mode == 4;
+        if (valu && mfma_mode && d.M >= 64 && d.N >= 64 && d.K >= (int64_t(1) << 15)) {
+            const int64_t tiles = d.batch * ceil_div(d.M, 128) * ceil_div(d.N, 128);
+            int64_t want = 1;
+            while (tiles * want < 512) want *= 2;
+            int64_t chunks = 1;
+            size_t n_chunk_modes = 0;
+            while (n_chunk_modes < gemm_common.size()) {
+                const int64_t e = ext_of(gemm_common[n_chunk_modes]);
+                if (chunks * e > want || d.K / (chunks * e) < 2048) break;
+                chunks *= e;
+                n_chunk_modes++;
+            }
+            if (chunks > 1) {
+                const int64_t k2 = d.K / chunks, b2 = d.batch * chunks;
+                const GemmChoice c2 = tuned_gemm_choice(mode, d.M, d.N, k2, b2, nullptr);
+                if ((c2.kernel == kGemmF32Mfma || c2.kernel == kGemmF64Mfma ||
+                     c2.kernel == kGemmF16x4m) &&
+                    gemm_choice_valid(mode, d.M, d.N, k2, b2, c2)) {
+                    std::vector<int32_t> chunk_modes(gemm_common.begin(),
+                                                     gemm_common.begin() + n_chunk_modes);
+                    gemm_common.erase(gemm_common.begin(), gemm_common.begin() + n_chunk_modes);
+                    gemm_batch = chunk_modes;
+                    gemm_batch.insert(gemm_batch.end(), batch.begin(), batch.end());
+                    d.chunks = chunks;
+                    d.batch = b2;
+                    d.K = k2;
+                    d.gemm_kernel = c2.kernel;
+                    d.gemm_variant = c2.variant;
+                    d.pa = plan_reorder(A, EA, cat(gemm_batch, left, gemm_common));
+                    d.pb = plan_reorder(B, EB, cat(gemm_batch, right, gemm_common));
+                    d.need_a = d.pa.kind != PermutePlan::kCopy;
+                    d.need_b = d.pb.kind != PermutePlan::kCopy;
+                }
+            }
+        }
+    }
     if (d.gemm_kernel == kGemmF16x4m) {
         // fold a permutation into the split pass when the rows need no
         // padding (the split rows are then the permuted rows)
-        const auto &common = best_opt == 0 ? common_a : common_b;
         if (d.need_a && d.M % kSplitRowPad == 0)
-            d.fuse_a = plan_split_gather(A, EA, cat(batch, left, common), d.K, d.ga);
+            d.fuse_a = plan_split_gather(A, EA, cat(gemm_batch, left, gemm_common), d.K, d.ga);
         if (d.need_b && d.N % kSplitRowPad == 0)
-            d.fuse_b = plan_split_gather(B, EB, cat(batch, right, common), d.K, d.gb);
+            d.fuse_b = plan_split_gather(B, EB, cat(gemm_batch, right, gemm_common), d.K, d.gb);
     }
 
     // Small steps on the one-thread-per-output or the one-block-per-output
@@ -443,6 +489,10 @@ void layout_workspace(ContractionDesc &d)
     if (d.gemm_kernel == kGemmSplitK) {
         d.off_partials = off;
         off = align256(off + 16 * (size_t)splitk_partials(d.batch * d.M * d.N, d.K));
+    }
+    if (d.chunks > 1) {
+        d.off_chunks = off;
+        off = align256(off + eb * (size_t)d.batch * (size_t)d.M * (size_t)d.N);
     }
     d.workspace = off;
 }

@@ -41,8 +41,14 @@ struct ContractionDesc {
     FusedSmall fs;
     int gemm_kernel = kGemmSmall;
     int gemm_variant = 0; // tile variant of gemm_kernel (GemmChoice)
+    // K split into `chunks` pieces that run as a batch of the MFMA GEMM
+    // (leading chunk modes), then summed: a step with many outputs and a
+    // long K whose tuned kernel is a VALU one.  batch and K above are then
+    // the GEMM's (chunks * result batch, K / chunks); size_c is the result's.
+    int64_t chunks = 1;
     // Workspace layout (bytes, 256-aligned offsets).
     size_t off_at = 0, off_bt = 0, off_ct = 0, off_sa = 0, off_sb = 0, off_partials = 0;
+    size_t off_chunks = 0; // the GEMM's per-chunk results (chunks > 1)
     // split-FP16 row maxima and scales of A and B (batch * padded rows each)
     size_t off_rmax_a = 0, off_scale_a = 0, off_rmax_b = 0, off_scale_b = 0;
     int64_t split_a_plane = 0, split_b_plane = 0;

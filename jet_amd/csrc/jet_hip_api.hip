@@ -156,6 +156,9 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         Bt = ws + d.off_bt;
     }
     void *Ct = d.need_c ? static_cast<void *>(ws + d.off_ct) : c;
+    // K split into batched chunks: the GEMM writes each chunk's result,
+    // then they are summed into Ct.
+    void *Cg = d.chunks > 1 ? static_cast<void *>(ws + d.off_chunks) : Ct;
     if (d.gemm_kernel == kGemmBf16x3 || d.gemm_kernel == kGemmBf16x4m ||
         d.gemm_kernel == kGemmF16x4m) {
         const int planes = d.gemm_kernel == kGemmBf16x3 ? 6 : 4;
@@ -178,17 +181,17 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
                 launch_split_f16_gather(b, d.gb, sb, s);
             else
                 launch_split_f16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
-            launch_gemm_f16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s,
+            launch_gemm_f16x4m(sa, sb, d.K, Cg, d.M, d.N, d.N, d.M * d.N, d.batch, false, false, s,
                                d.gemm_variant);
         }
         else {
             launch_split_bf16(At, d.M, d.K, d.K, d.M * d.K, d.batch, sa, s);
             launch_split_bf16(Bt, d.N, d.K, d.K, d.N * d.K, d.batch, sb, s);
             if (planes == 6)
-                launch_gemm_bf16x3(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false,
+                launch_gemm_bf16x3(sa, sb, d.K, Cg, d.M, d.N, d.N, d.M * d.N, d.batch, false,
                                    false, s);
             else
-                launch_gemm_bf16x4m(sa, sb, d.K, Ct, d.M, d.N, d.N, d.M * d.N, d.batch, false,
+                launch_gemm_bf16x4m(sa, sb, d.K, Cg, d.M, d.N, d.N, d.M * d.N, d.batch, false,
                                     false, s, d.gemm_variant);
         }
     }
@@ -196,7 +199,7 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         GemmArgs g;
         g.A = At;
         g.B = Bt;
-        g.C = Ct;
+        g.C = Cg;
         g.M = d.M;
         g.N = d.N;
         g.K = d.K;
@@ -210,6 +213,7 @@ void run_contraction(const ContractionDesc &d, const void *a, const void *b, voi
         g.variant = d.gemm_variant;
         launch_gemm(d.gemm_kernel, eb, g, s, ws + d.off_partials);
     }
+    if (d.chunks > 1) launch_sum_chunks(eb, Cg, Ct, d.size_c, d.chunks, s);
     if (d.need_c) launch_permute(d.pc, eb, Ct, c, s);
 }
 
@@ -491,8 +495,8 @@ int jet_hip_plan_info(jet_hip_plan_t plan, double *flops, int64_t *m, int64_t *n
         if (flops) *flops = plan->d.flops;
         if (m) *m = plan->d.M;
         if (n) *n = plan->d.N;
-        if (k) *k = plan->d.K;
-        if (batch) *batch = plan->d.batch;
+        if (k) *k = plan->d.K * plan->d.chunks;
+        if (batch) *batch = plan->d.batch / plan->d.chunks;
         if (gemm_kernel) *gemm_kernel = plan->d.gemm_kernel;
     });
 }
@@ -1085,8 +1089,8 @@ int jet_hip_network_step_info(jet_hip_network_t net, int step, int64_t *m, int64
         const ContractionDesc &d = net->plan.steps[step].desc;
         if (m) *m = d.M;
         if (n) *n = d.N;
-        if (k) *k = d.K;
-        if (batch) *batch = d.batch;
+        if (k) *k = d.K * d.chunks;
+        if (batch) *batch = d.batch / d.chunks;
         if (gemm_kernel) *gemm_kernel = d.gemm_kernel;
         if (permutes) *permutes = (d.need_a ? 1 : 0) | (d.need_b ? 2 : 0) | (d.need_c ? 4 : 0);
     });

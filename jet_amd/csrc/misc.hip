@@ -103,6 +103,26 @@ __global__ __launch_bounds__(256) void convert_kernel(const S *__restrict__ in,
 int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
 
 
+// out[i] = sum over c of part[c * n + i] (c ascending, accumulated in FP64):
+// the per-chunk results of a GEMM whose K was split into batched chunks.
+template <class C2>
+__global__ __launch_bounds__(256) void sum_chunks_kernel(const C2 *__restrict__ part,
+                                                         C2 *__restrict__ out, int64_t n,
+                                                         int64_t chunks)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        double re = 0.0, im = 0.0;
+        for (int64_t c = 0; c < chunks; c++) {
+            const C2 v = part[c * n + i];
+            re += double(v.x);
+            im += double(v.y);
+        }
+        out[i].x = decltype(out[i].x)(re);
+        out[i].y = decltype(out[i].y)(im);
+    }
+}
+
 typedef _Float16 mfma_h8 __attribute__((ext_vector_type(8)));
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
@@ -210,6 +230,23 @@ void launch_reduce(int elem_bytes, const void *x, int64_t n, double *out,
     else
         hipLaunchKernelGGL(reduce_kernel<double2>, dim3(1), dim3(1024), 0, stream,
                            static_cast<const double2 *>(x), n, out);
+    JH_HIP(hipGetLastError());
+}
+
+void launch_sum_chunks(int elem_bytes, const void *part, void *out, int64_t n, int64_t chunks,
+                       hipStream_t stream)
+{
+    JH_REQUIRE(n >= 1 && chunks >= 1, "Chunk sum: bad sizes.");
+    prof::Scope ps(prof::kAdd, stream, 2.0 * double(n) * double(chunks),
+                   double(elem_bytes) * double(n) * double(chunks + 1));
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(sum_chunks_kernel<float2>, dim3(grid_for(n)), dim3(256), 0, stream,
+                           static_cast<const float2 *>(part), static_cast<float2 *>(out), n,
+                           chunks);
+    else
+        hipLaunchKernelGGL(sum_chunks_kernel<double2>, dim3(grid_for(n)), dim3(256), 0, stream,
+                           static_cast<const double2 *>(part), static_cast<double2 *>(out), n,
+                           chunks);
     JH_HIP(hipGetLastError());
 }
 

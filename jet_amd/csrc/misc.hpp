@@ -57,6 +57,10 @@ void launch_reduce(int elem_bytes, const void *x, int64_t n, double *out,
 // complex64 <-> complex128 (rounding to nearest when narrowing).
 void launch_convert(int src_bytes, int dst_bytes, const void *in, void *out,
                     int64_t n, hipStream_t stream);
+// out[i] = sum over c < chunks of part[c * n + i], in FP64 (complex64 or
+// complex128 by elem_bytes).
+void launch_sum_chunks(int elem_bytes, const void *part, void *out, int64_t n,
+                       int64_t chunks, hipStream_t stream);
 // The FP16 MFMA rate loop on `blocks` work-groups of four waves: sink holds
 // blocks * 256 floats, clocks 8 * blocks (shader-clock and constant-rate
 // timer deltas per wave); rounds is a multiple of 4.

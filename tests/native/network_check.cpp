@@ -6,6 +6,7 @@
 // overlap, and that each step's operands are the results of earlier steps
 // or leaves.
 #include "../../jet_amd/csrc/network_plan.hpp"
+#include "../../jet_amd/csrc/tuning.hpp"
 
 #include <cstring>
 #include <vector>
@@ -301,4 +302,35 @@ extern "C" int contraction_is_fused(int ra, const int32_t *ma, const int64_t *ea
     const ContractionDesc d = plan_contraction(8, 0, 0, ra, ma, ea, rb, mb, eb, rc, mc);
     *workspace = d.workspace;
     return d.fused_small ? 1 : 0;
+}
+
+// plan_contraction's GEMM for complex64 (mode 0 or 4) or complex128 (mode
+// 2) operands, with the table replaced by `n_entries` entries (7 ints each;
+// n_entries < 0 keeps the compiled-in table): kernel, chunks (K split into
+// batched chunks and summed), the GEMM's batch and K, the workspace bytes and
+// the chunk buffer's offset.
+extern "C" int contraction_gemm_plan(int mode, int n_entries, const int32_t *entries, int ra,
+                                     const int32_t *ma, const int64_t *ea, int rb,
+                                     const int32_t *mb, const int64_t *eb, int rc,
+                                     const int32_t *mc, int64_t *out)
+{
+    if (n_entries >= 0) {
+        std::vector<TuneEntry> t(n_entries);
+        for (int i = 0; i < n_entries; i++) {
+            const int32_t *e = entries + 7 * i;
+            t[i] = TuneEntry{e[0], e[1], e[2], e[3], e[4], e[5], e[6]};
+        }
+        tuning_set(t);
+    }
+    const int elem = mode == 2 ? 16 : 8;
+    const ContractionDesc d = plan_contraction(elem, mode, 0, ra, ma, ea, rb, mb, eb, rc, mc);
+    if (n_entries >= 0) tuning_reset();
+    out[0] = d.gemm_kernel;
+    out[1] = d.chunks;
+    out[2] = d.batch;
+    out[3] = d.K;
+    out[4] = (int64_t)d.workspace;
+    out[5] = (int64_t)d.off_chunks;
+    out[6] = d.size_c;
+    return 0;
 }
