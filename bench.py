@@ -873,7 +873,9 @@ def run_c3(args):
     14, 6x6 depth 12, 6x7 depth 10; bitstrings seeded) plus a range of
     --c3-slices slices of the c2 amplitude, all executions captured in one
     hipGraph (jet_amd.batch.UnevenBatch; each amplitude job's form, one
-    execution per bitstring or all bitstrings batched, chosen by timing).
+    execution per bitstring or all bitstrings batched -- the stack padded
+    to a power of two or not -- chosen by timing; flops count the requested
+    amplitudes only).
     One launch is timed with the --split GEMM; the same batch in complex128
     on FP64 MFMA (the c2 part: its first execution only) gives the
     per-amplitude errors."""
@@ -910,7 +912,9 @@ def run_c3(args):
         if isinstance(job, SliceJob):
             flops += part.net.flops * job.executions
         else:
-            flops += part.net.flops * (len(part.execs))
+            # requested amplitudes only: a stack padded to a power of two
+            # does n_stack bitstrings' work for n results
+            flops += part.net.flops * len(part.execs) * part.n / part.n_stack
     e0, e1 = nat.Event(ctx), nat.Event(ctx)
     ctx.synchronize()
     e0.record()

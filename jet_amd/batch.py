@@ -16,14 +16,18 @@ Jobs:
       - "stacked": the bitstring index is a batch label on every output
         vector, so one execution computes all of them and every step's GEMM
         is batched over the bitstrings (the better form for small circuits,
-        whose single-amplitude GEMMs are too small for the MFMA tiles).
+        whose single-amplitude GEMMs are too small for the MFMA tiles);
+      - "stacked_pow2": the same with the stack padded to a power of two
+        (the last bitstring repeated, its extra results dropped), so that
+        every extent is a power of two: bit permutations and the fused
+        small-step pass instead of the generic odometer permutation.
   * ``SliceJob``: a contiguous range of slices of a sliced network
     (``SlicedContraction``'s executions), summed afterwards by the FP64
     compensated reduction.
 
 Autotuning: for each amplitude job whose stacked plan fits the memory
-budget, both forms are captured alone and timed (median of a few replays);
-the faster one is kept.  ``tuning`` records the measured times.
+budget, the forms are captured alone and timed (median of a few replays);
+the fastest one is kept.  ``tuning`` records the measured times.
 """
 from __future__ import annotations
 
@@ -86,7 +90,7 @@ class AmplitudeJob:
     circuit: Circuit
     bitstrings: Sequence[Sequence[int]]
     path: Sequence[Tuple[int, int]]
-    form: str = "auto"      # "auto" (tuned), "each" or "stacked"
+    form: str = "auto"      # "auto" (tuned), "each", "stacked" or "stacked_pow2"
 
 
 class _Amplitudes:
@@ -100,12 +104,17 @@ class _Amplitudes:
         nq = circ.n_qubits
         bits = [tuple(int(x) for x in b) for b in job.bitstrings]
         self.n = len(bits)
+        if form == "stacked_pow2":
+            n2 = 1 << max(0, (self.n - 1).bit_length())
+            bits = bits + [bits[-1]] * (n2 - self.n)
+            form = "stacked"
+        self.n_stack = len(bits)
         net = amplitude_network(circ, bits[0], dtype=np.complex128)
         n_leaves = len(net.indices)
         first_out = n_leaves - nq           # output vectors are the last nq leaves
         self.bufs: List[nat.DeviceBuffer] = [_upload(d.astype(self.dtype), ctx)
                                              for d in net.data[:first_out]]
-        self.out = nat.DeviceBuffer(max(16, self.n * self.dtype.itemsize), ctx)
+        self.out = nat.DeviceBuffer(max(16, self.n_stack * self.dtype.itemsize), ctx)
         if form == "each":
             self.net = DeviceNetwork(net.indices, net.shapes, job.path, [], dtype=self.dtype,
                                      precision=precision, ctx=ctx)
@@ -127,8 +136,8 @@ class _Amplitudes:
             for q in range(nq):
                 k = first_out + q
                 idx[k] = [lab] + idx[k]
-                shp[k] = [self.n] + shp[k]
-                v = np.zeros((self.n, 2), self.dtype)
+                shp[k] = [self.n_stack] + shp[k]
+                v = np.zeros((self.n_stack, 2), self.dtype)
                 for i, b in enumerate(bits):
                     v[i, b[q]] = 1
                 outs.append(_upload(v.reshape(-1), ctx))
@@ -140,9 +149,9 @@ class _Amplitudes:
             raise JetError(f"Unknown amplitude form '{form}'.")
 
     def results(self) -> np.ndarray:
-        host = np.empty(self.n, self.dtype)
+        host = np.empty(self.n_stack, self.dtype)
         self.out.download(host)
-        return host
+        return host[:self.n]
 
 
 def stacked_bytes(job: AmplitudeJob, dtype=np.complex64, precision=None) -> int:
@@ -200,10 +209,20 @@ class UnevenBatch:
         self.batch = Batch(self._executions(), self.ctx)
 
     def _amplitude_part(self, job: AmplitudeJob, budget: float, reps: int):
-        forms = [job.form] if job.form != "auto" else ["each", "stacked"]
-        if "stacked" in forms and job.form == "auto" and \
-                stacked_bytes(job, self.dtype, self.precision) > budget:
-            forms = ["each"]
+        n = len(job.bitstrings)
+        pow2 = n & (n - 1) == 0
+        forms = [job.form] if job.form != "auto" else \
+            (["each", "stacked"] + ([] if pow2 else ["stacked_pow2"]))
+        if job.form == "auto":
+            padded = AmplitudeJob(job.circuit,
+                                  list(job.bitstrings) +
+                                  [job.bitstrings[-1]] * ((1 << max(0, (n - 1).bit_length())) - n),
+                                  job.path)
+            if stacked_bytes(job, self.dtype, self.precision) > budget:
+                forms = [f for f in forms if f == "each"]
+            elif "stacked_pow2" in forms and \
+                    stacked_bytes(padded, self.dtype, self.precision) > budget:
+                forms.remove("stacked_pow2")
         if len(forms) == 1:
             self.tuning.append({})
             return _Amplitudes(job, forms[0], self.dtype, self.precision, self.ctx)

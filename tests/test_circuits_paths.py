@@ -86,9 +86,13 @@ def _check_lib():
     deps = srcs + [jb.TUNING_INC]
     if not os.path.exists(CHECK_LIB) or os.path.getmtime(CHECK_LIB) < max(map(os.path.getmtime, deps)):
         os.makedirs(os.path.dirname(CHECK_LIB), exist_ok=True)
+        # built under a per-process name and renamed into place, so that
+        # parallel test workers never load a half-written library
+        tmp = f"{CHECK_LIB}.{os.getpid()}.tmp"
         subprocess.run(["g++", "-std=c++17", "-O2", "-fcx-limited-range", "-fPIC", "-shared",
-                        "-I" + os.path.join(ROOT, "include"), "-I" + jb.BUILD, "-o", CHECK_LIB]
+                        "-I" + os.path.join(ROOT, "include"), "-I" + jb.BUILD, "-o", tmp]
                        + srcs, check=True)
+        os.replace(tmp, CHECK_LIB)
     return ctypes.CDLL(CHECK_LIB)
 
 

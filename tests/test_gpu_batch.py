@@ -92,8 +92,10 @@ def test_uneven_batch_one_launch(dtype, precision, tol):
 
 
 def test_both_amplitude_forms_agree():
+    """Every amplitude form (one execution per bitstring, the stacked batch,
+    the stack padded to a power of two) gives the same amplitudes."""
     jobs, refs, _ = _jobs()
-    for form in ("each", "stacked"):
+    for form in ("each", "stacked", "stacked_pow2"):
         js = [AmplitudeJob(j.circuit, j.bitstrings, j.path, form) for j in jobs[:3]]
         ub = UnevenBatch(js, dtype=np.complex128, precision="fp64")
         assert ub.forms() == [form] * 3
